@@ -1,0 +1,256 @@
+"""simulate and run_rt — src/MonteCarloRadiativeTransfer.jl:26-291.
+
+``simulate(cfg, g, I0)`` is the drop-in boundary: it hands the solve to
+libmcrt_hip.so (``mcrt_simulate``) and returns the reference's 7-tuple
+``(Adt, Aabs, Aref, Atrans, Arem, Irt, Icb)`` normalised by N, with Irt and
+Icb as (4, Nθ, Nϕ) and (4, Nθb, Nϕb) arrays indexed like Julia's.  There is
+no CPU fallback: without the library or a GPU the call raises.
+
+Beyond the reference's arguments, ``seed`` keys the Philox stream and
+``precision`` picks the arithmetic form ("fp64_plain", "fp64", "mixed").
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import threading
+
+import numpy as np
+
+from . import _abi
+from .config import Config
+from .geometries import AbstractGeometry, PlaneGeometry, SphereGeometry
+from .geometries import to_struct as geom_struct
+
+DEFAULT_SEED = 42
+
+
+class Context:
+    """One HIP device context of the library (stream, scratch)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _abi.load_library()
+        h = C.c_void_p()
+        _abi.check(self.lib.mcrt_context_create(int(device), C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.mcrt_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_ctx_lock = threading.Lock()
+_contexts: dict[int, Context] = {}
+
+
+def get_context(device: int = 0) -> Context:
+    with _ctx_lock:
+        if device not in _contexts:
+            _contexts[device] = Context(device)
+        return _contexts[device]
+
+
+def marshal(cfg: Config, g: AbstractGeometry, I0):
+    """ctypes structs for (cfg, g, I0) plus the objects that keep them alive."""
+    keep: list = []
+    cs = cfg.to_struct(keep)
+    gs = geom_struct(g, keep)
+    i0 = _abi.f64(I0)
+    if i0.shape != (4,):
+        raise ValueError("I0 must be a Stokes 4-vector")
+    keep.append(i0)
+    return cs, gs, i0, keep
+
+
+def _shape_outputs(cfg: Config, A, Irt, Icb):
+    Irt = Irt.reshape(cfg.Nphi, cfg.Ntheta, 4).transpose(2, 1, 0)
+    Icb = Icb.reshape(cfg.Nphib, cfg.Nthetab, 4).transpose(2, 1, 0)
+    return (A[0], A[1], A[2], A[3], A[4], Irt, Icb)
+
+
+def simulate(cfg: Config, g: AbstractGeometry, I0, *, use_gpu: bool = True, seed: int = DEFAULT_SEED,
+             precision: str = "fp64", device: int = 0, stats: dict | None = None):
+    """simulate(cfg, g, I₀; use_gpu) — MonteCarloRadiativeTransfer.jl:26-85."""
+    if not use_gpu:
+        raise NotImplementedError(
+            "use_gpu=false is the reference's Julia CPU path; this package provides the GPU path")
+    g.validate()
+    ctx = get_context(device)
+    cs, gs, i0, keep = marshal(cfg, g, I0)
+    A = np.zeros(5)
+    Irt = np.zeros(4 * cfg.Ntheta * cfg.Nphi)
+    Icb = np.zeros(max(1, 4 * cfg.Nthetab * cfg.Nphib))
+    st = _abi.mcrt_stats()
+    rc = ctx.lib.mcrt_simulate(ctx.handle, C.byref(cs), C.byref(gs), _abi.dptr(i0),
+                               C.c_uint64(seed & (2**64 - 1)), _abi.PRECISIONS[precision],
+                               _abi.dptr(A), _abi.dptr(Irt), _abi.dptr(Icb), C.byref(st))
+    _abi.check(rc, ctx.handle)
+    if stats is not None:
+        stats.update(st.as_dict())
+    return _shape_outputs(cfg, A, Irt, Icb[: 4 * cfg.Nthetab * cfg.Nphib])
+
+
+class Plan:
+    """A resident solve: inputs staged in HBM once, launched any number of times."""
+
+    def __init__(self, cfg: Config, g: AbstractGeometry, I0, *, seed: int = DEFAULT_SEED,
+                 precision: str = "fp64", device: int = 0):
+        g.validate()
+        self.cfg = cfg
+        self.ctx = get_context(device)
+        cs, gs, i0, keep = marshal(cfg, g, I0)
+        h = C.c_void_p()
+        rc = self.ctx.lib.mcrt_plan_create(self.ctx.handle, C.byref(cs), C.byref(gs), _abi.dptr(i0),
+                                           C.c_uint64(seed & (2**64 - 1)),
+                                           _abi.PRECISIONS[precision], C.byref(h))
+        _abi.check(rc, self.ctx.handle)
+        self.handle = h
+
+    def launch(self) -> None:
+        _abi.check(self.ctx.lib.mcrt_plan_launch(self.handle), self.ctx.handle)
+
+    def wait(self) -> None:
+        _abi.check(self.ctx.lib.mcrt_plan_wait(self.handle), self.ctx.handle)
+
+    def kernel_ms(self) -> float:
+        ms = C.c_double()
+        _abi.check(self.ctx.lib.mcrt_plan_kernel_ms(self.handle, C.byref(ms)), self.ctx.handle)
+        return ms.value
+
+    def read(self, stats: dict | None = None):
+        cfg = self.cfg
+        A = np.zeros(5)
+        Irt = np.zeros(4 * cfg.Ntheta * cfg.Nphi)
+        Icb = np.zeros(max(1, 4 * cfg.Nthetab * cfg.Nphib))
+        st = _abi.mcrt_stats()
+        _abi.check(self.ctx.lib.mcrt_plan_read(self.handle, _abi.dptr(A), _abi.dptr(Irt),
+                                               _abi.dptr(Icb), C.byref(st)), self.ctx.handle)
+        if stats is not None:
+            stats.update(st.as_dict())
+        return _shape_outputs(cfg, A, Irt, Icb[: 4 * cfg.Nthetab * cfg.Nphib])
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.ctx.lib.mcrt_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ run_rt
+def mueller_sphere(cfg: Config, runs):
+    """Mueller assembly of run_rt(::SphereGeometry), :87-184, from simulate outputs.
+
+    runs: list of (I0, simulate-tuple) for the columns of initial_I(s).
+    Returns (MRT[4,4,Nθ], MCB[4,4,Nθb], energy dict)."""
+    if cfg.Nphi % 8 or cfg.Nphib % 8:
+        raise ValueError("the sphere Mueller assembly needs Nϕ and Nϕb divisible by 8")
+    MRT = np.zeros((4, 4, cfg.Ntheta))
+    MCB = np.zeros((4, 4, cfg.Nthetab))
+    NFI = [i * cfg.Nphi // 8 for i in range(8)]
+    NFIB = [i * cfg.Nphib // 8 for i in range(8)]
+    G = dict(dt=0.0, abs=0.0, ref=0.0, rem=0.0)
+    for I, (Adt, Aabs, Aref, _At, Arem, IRT, ICB) in runs:
+        if I[3] == 0:  # linear
+            G["dt"] += Adt; G["abs"] += Aabs; G["ref"] += Aref; G["rem"] += Arem
+            for M, X, F in ((MRT, IRT, NFI), (MCB, ICB, NFIB)):
+                M[:, 0, :] += (X[:, :, F[0]] + X[:, :, F[2]] + X[:, :, F[4]] + X[:, :, F[6]]) * 0.25
+                M[:, 1, :] += (-X[:, :, F[0]] + X[:, :, F[2]] - X[:, :, F[4]] + X[:, :, F[6]]) * 0.25
+                M[:, 2, :] += (X[:, :, F[1]] - X[:, :, F[3]] + X[:, :, F[5]] - X[:, :, F[7]]) * 0.25
+        else:
+            for M, X, F, n in ((MRT, IRT, NFI, cfg.Nphi), (MCB, ICB, NFIB, cfg.Nphib)):
+                M[2, 3, :] -= (X[2, :, F[0]] + X[2, :, F[4]]) * 0.5
+                M[3, 3, :] -= X[3, :, :].sum(axis=1) / n
+    renorm = 1.0 / (G["dt"] + G["ref"])
+    MRT *= renorm / cfg.inorm[None, None, :]
+    MCB *= renorm
+    return MRT, MCB, G
+
+
+def mueller_plane(cfg: Config, runs):
+    """Mueller assembly of run_rt(::PlaneGeometry), :186-291.
+
+    Returns (MRT[4,4,Nθ,Nϕ], MCB[4,4,Nθb,Nϕb], energy dict)."""
+    MRT = np.zeros((4, 4, cfg.Ntheta, cfg.Nphi))
+    MCB = np.zeros((4, 4, cfg.Nthetab, cfg.Nphib))
+    G = dict(dt=0.0, abs=0.0, ref=0.0, trans=0.0, rem=0.0)
+    eps = np.finfo(float).eps
+    for I, (Adt, Aabs, Aref, Atrans, Arem, IRT, ICB) in runs:
+        k1, k2 = 0, 0.0
+        for j1 in range(1, 4):
+            if abs(I[j1]) > 1.0 - eps:
+                k1, k2 = j1, I[j1]
+                break
+        if k1 == 1:
+            G["dt"] += Adt / 2; G["abs"] += Aabs / 2; G["ref"] += Aref / 2
+            G["trans"] += Atrans / 2; G["rem"] += Arem / 2
+            MRT[:, 0] += IRT / 2.0
+            MCB[:, 0] += ICB / 2.0
+        MRT[:, k1] += k2 * IRT / 2.0
+        MCB[:, k1] += k2 * ICB / 2.0
+    norm = np.abs(1.0 / (4.0 * cfg.costheta * cfg.inorm))
+    MRT *= norm[None, None, :, None]
+    norm1 = 1.0 / np.abs(4.0 * cfg.costhetab)
+    MCB *= norm1[None, None, :, None]
+    return MRT, MCB, G
+
+
+def _frame(columns: dict):
+    try:
+        import pandas as pd
+        return pd.DataFrame(columns)
+    except ImportError:  # pragma: no cover
+        return columns
+
+
+def run_rt(cfg: Config, g: AbstractGeometry, *, use_gpu: bool = True, seed: int = DEFAULT_SEED,
+           precision: str = "fp64", device: int = 0, verbose: bool = False):
+    """run_rt(cfg, g; use_gpu) — MonteCarloRadiativeTransfer.jl:87-291.
+
+    Runs simulate once per incident Stokes column of initial_I(g) and returns
+    (SM_rt, SM_cb) as DataFrames with columns θ[, ϕ], s11 … s44.  Column i
+    of initial_I is solved with the Philox key seed + i."""
+    g.validate()
+    IS = g.initial_I()
+    runs = []
+    for i in range(IS.shape[1]):
+        I = IS[:, i].copy()
+        if verbose:
+            print(f"I = {I}")
+        runs.append((I, simulate(cfg, g, I, use_gpu=use_gpu, seed=seed + i, precision=precision,
+                                 device=device)))
+    deg = 180.0 / math.pi
+    if isinstance(g, SphereGeometry):
+        MRT, MCB, G = mueller_sphere(cfg, runs)
+        rt = {"θ": deg * np.arccos(cfg.costheta)}
+        cb = {"θ": deg * np.arccos(cfg.costhetab)}
+        for i in range(4):
+            for j in range(4):
+                rt[f"s{i + 1}{j + 1}"] = MRT[i, j]
+                cb[f"s{i + 1}{j + 1}"] = MCB[i, j]
+    else:
+        MRT, MCB, G = mueller_plane(cfg, runs)
+        rt = {"θ": np.tile(deg * np.arccos(cfg.costheta), cfg.Nphi),
+              "ϕ": np.repeat(deg * cfg.phi, cfg.Ntheta)}
+        cb = {"θ": np.tile(deg * np.arccos(cfg.costhetab), cfg.Nphib),
+              "ϕ": np.repeat(deg * cfg.phib, cfg.Nthetab)}
+        for i in range(4):
+            for j in range(4):
+                rt[f"s{i + 1}{j + 1}"] = MRT[i, j].reshape(-1, order="F")
+                cb[f"s{i + 1}{j + 1}"] = MCB[i, j].reshape(-1, order="F")
+    if verbose:
+        for k, v in G.items():
+            print(f"{k} (%) = {100 * v:.4f}")
+    return _frame(rt), _frame(cb)

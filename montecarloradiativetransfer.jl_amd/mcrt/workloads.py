@@ -1,0 +1,64 @@
+"""Named workloads of the benchmark and the parity tests.
+
+  plane_rayleigh_tau1   plane slab τ=1, ω=1, Rayleigh, normal incidence, RT only,
+                        ≤1000 orders, 180 polar bins of 1° × 48 azimuths, seed 42
+  sphere_mie_cb         sphere of optical radius 5, Mie spheres r=0.5 µm at
+                        λ=0.55 µm (x≈5.712), m=1.5+0.001i, ν=0.1, F/S tabulated on
+                        1801 angles (0.1°), random_points=1000; RT 180 × 1° polar
+                        bins × 48 azimuths; CB θb = 0:0.05°:5° (101) × 8; seed 7
+The ray counts of the full workloads are 1e6 and 1e7; tests use fewer rays of
+the same configuration.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .config import Config, midpoint_polar_grid
+from .geometries import Layer, PlaneGeometry, SphereGeometry
+from .scatterers import MieScatterer, RayleighScatterer
+
+_MIE_CACHE: dict = {}
+
+
+def c2_mie(Ntheta: int = 1801) -> MieScatterer:
+    key = ("c2", Ntheta)
+    if key not in _MIE_CACHE:
+        _MIE_CACHE[key] = MieScatterer.create(0.5, 1.5 + 0.001j, 0.55, 0.1, Ntheta=Ntheta,
+                                              random_points=1000)
+    return _MIE_CACHE[key]
+
+
+def plane_rayleigh_tau1(n_rays: int = 10**6, *, compute_cb: bool = False, gl_grid: bool = False,
+                        max_orders: int = 1000, tau: float = 1.0, omega: float = 1.0,
+                        Nphi: int = 48, Ntheta: int = 180):
+    """Plane-parallel slab, Rayleigh, normal incidence (θ = π)."""
+    sca = RayleighScatterer(omega, 1.0)
+    g = PlaneGeometry(math.pi, 0.0, [Layer(d=tau * sca.kl, scatterer=sca, tau=tau, omega=omega)])
+    kw = dict(number_of_rays=n_rays, maximum_scattering_times=max_orders, compute_cb=compute_cb,
+              Ntheta=Ntheta, Nphi=Nphi, minimum_intensity=1e-6, tau0=50.0)
+    if not gl_grid:
+        ct, st, w = midpoint_polar_grid(g, Ntheta, Nphi)
+        kw.update(costheta=ct, sintheta=st, inorm=w)
+    cfg = Config(g, **kw)
+    return cfg, g, 42
+
+
+def sphere_mie_cb(n_rays: int = 10**7, *, Ntheta_table: int = 1801, Nphib: int = 8,
+                  thetab_deg=None, compute_cb: bool = True, max_orders: int = 10000,
+                  tau: float = 5.0, Nphi: int = 48, Ntheta: int = 180):
+    """Sphere of Mie scatterers with CB near exact backscattering."""
+    sca = c2_mie(Ntheta_table)
+    g = SphereGeometry(R=tau * sca.kl, scatterer=sca)
+    if thetab_deg is None:
+        thetab_deg = np.arange(101) * 0.05
+    ct, st, w = midpoint_polar_grid(g, Ntheta, Nphi)
+    cfg = Config(g, number_of_rays=n_rays, maximum_scattering_times=max_orders,
+                 compute_cb=compute_cb, Ntheta=Ntheta, Nphi=Nphi, costheta=ct, sintheta=st,
+                 inorm=w, thetab=np.deg2rad(np.asarray(thetab_deg, dtype=float)), Nphib=Nphib,
+                 minimum_intensity=1e-6, tau0=50.0)
+    return cfg, g, 7
+
+
+WORKLOADS = {"plane_rayleigh_tau1": plane_rayleigh_tau1, "sphere_mie_cb": sphere_mie_cb}

@@ -1,0 +1,65 @@
+"""GPU parity: every precision form against the oracle on the same Philox stream.
+
+The oracle (oracle/liboracle.so) restates the reference CPU path; the GPU
+forms consume the same counter-based draws, so their per-ray histories agree
+with it up to rounding.  Acceptance is statistical (binomial count of |z|>3
+plus χ²/dof, with σ from the oracle's batch means), plus exact energy
+telescoping.  Sizes are chosen so the oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+import mcrt
+import oracle
+from mcrt import workloads as W
+from statcheck import compare, flat, oracle_flat
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ["fp64_plain", "fp64", "mixed"]
+
+
+def _cases():
+    return {
+        "plane_rayleigh": W.plane_rayleigh_tau1(2048),
+        "plane_rayleigh_cb": W.plane_rayleigh_tau1(1024, compute_cb=True),
+        "sphere_mie_cb": W.sphere_mie_cb(512, Ntheta_table=1801),
+    }
+
+
+_ORACLE = {}
+
+
+def _oracle(name, cfg, g, seed, I0):
+    key = (name, tuple(I0))
+    if key not in _ORACLE:
+        _ORACLE[key] = oracle.Solve(cfg, g).simulate(I0, seed, nbatch=32)
+    return _ORACLE[key]
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("case", ["plane_rayleigh", "plane_rayleigh_cb", "sphere_mie_cb"])
+def test_form_vs_oracle(case, form):
+    cfg, g, seed = _cases()[case]
+    I0 = [1.0, -1.0, 0.0, 0.0]
+    print(f"case={case} form={form} N={cfg.number_of_rays}", flush=True)
+    ref = _oracle(case, cfg, g, seed, I0)
+    st = {}
+    try:
+        out = mcrt.simulate(cfg, g, I0, seed=seed, precision=form, stats=st)
+    except mcrt._abi.MCRTError as e:  # form not built yet
+        if e.code == mcrt._abi.MCRT_ERR_UNSUPPORTED:
+            pytest.skip(str(e))
+        raise
+    a = flat(out)
+    b = oracle_flat(ref)
+    sig = oracle.sigma_of_mean(ref) * np.sqrt(2.0)
+    # energy budget telescopes to 1 per ray
+    assert abs(sum(out[:5]) - 1.0) < 1e-9
+    assert st["rays"] == cfg.number_of_rays
+    assert st["nonfinite"] == 0
+    r = compare(a, b, sig, floor=1e-15, label=f"{case}/{form}")
+    assert r["ok"], r
+    # same stream: event counts agree closely
+    ev_ref = ref["counters"][0]
+    assert abs(st["events"] - ev_ref) <= 0.02 * ev_ref + 5
