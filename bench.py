@@ -54,18 +54,18 @@ def run_form(cfg, g, seed, I0, form, steps, warmup):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sphere_mie_cb")
     ap.add_argument("--rays", type=int, default=None, help="override the workload's ray count")
     ap.add_argument("--form", default="auto", choices=["auto", "mixed", "fp64", "fp64_plain"])
     ap.add_argument("--compare", default="fp64,fp64_plain",
                     help="other forms timed beside the headline form")
-    ap.add_argument("--plain-sample", type=int, default=16384,
+    ap.add_argument("--plain-sample", type=int, default=8192,
                     help="rays of the plain FP64 comparison run")
     ap.add_argument("--fp64-sample", type=int, default=2_000_000,
                     help="rays of the FP64 comparison run")
-    ap.add_argument("--cpu-sample", type=int, default=2048,
+    ap.add_argument("--cpu-sample", type=int, default=8192,
                     help="rays of the oracle CPU baseline run")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

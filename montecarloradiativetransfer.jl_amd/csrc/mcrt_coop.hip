@@ -250,6 +250,74 @@ template <typename R> struct Slot {
   Fr<double> kbs;   // frame of -k₂
 };
 
+// One pass of the peel-off (cpu.jl:11-28) over CH register-resident bins per
+// thread and all E events of the iteration.  Thread tid owns the bins
+// lb = tid + kBlock·(c0 + i) of the θ-range [jb, jb + nrow) × Nϕ; the escape
+// sum of each event is wave-reduced and added by lane 0 to esc in chunk order.
+template <int CH, typename R, int GEO>
+MCRT_DEV void rt_pass(const CoopParams<R>& p, const ScaView<R>& sca, const R* rtrec, R* esc,
+                      double* slab, double* slabc, int E, int tid, int lane, int wave, int jb,
+                      int nrow, int nb, int c0, int flag, int slot2) {
+  R bk[CH][7];
+  int bidx[CH];
+  R acc[CH][4];
+#pragma unroll
+  for (int i = 0; i < CH; ++i) {
+    const int lb = tid + kBlock * (c0 + i);
+    const bool valid = lb < nb;
+    const int j1 = jb + lb % nrow, j2 = lb / nrow;
+    const int ob = j1 + p.Nt * j2;
+    bidx[i] = valid ? ob : -1;
+    const R* src = p.rt_bin + 8 * (valid ? ob : 0);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) bk[i][c] = src[c];
+    if (!valid) bk[i][5] = R(0);  // zero weight
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[i][c] = R(0);
+  }
+  const R tau2 = p.tau[0] * p.tau[0];
+  for (int e = 0; e < E; ++e) {
+    const R* rec = rtrec + kRtRec * e;
+    const int fl = (int)rec[15];
+    if (!(fl & flag)) continue;
+    R m[8], I[4], pos[3];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) m[c] = rec[c];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) I[c] = rec[8 + c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pos[c] = rec[12 + c];
+    R part = R(0);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      R o[4];
+      peel_I(sca, m, I, bk[i][0], bk[i][1], bk[i][2], bk[i][3], bk[i][4], o);
+      R t;
+      if (GEO == 0) {  // distance_to_boundary, plane.jl:97-105
+        t = bk[i][2] > R(0) ? -pos[0] * bk[i][6] : (pos[0] + pos[1]) * -bk[i][6];
+      } else {         // sphere.jl:69-73
+        R kx = bk[i][0] * pos[0] + bk[i][1] * pos[1] + bk[i][2] * pos[2];
+        R xx = pos[0] * pos[0] + pos[1] * pos[1] + pos[2] * pos[2];
+        t = -kx + M<R>::sqrt(kx * kx + tau2 - xx);
+      }
+      R w = t <= p.tau0 ? bk[i][5] * M<R>::exp(-t) : R(0);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[i][c] += o[c] * w;
+      part += o[0] * w;
+    }
+    part = wave_sum(part);
+    if (lane == 0) esc[(e * kWaves + wave) * 2 + slot2] += part;
+  }
+  // flush into the workgroup slab (bins are owned by one thread: no races)
+#pragma unroll
+  for (int i = 0; i < CH; ++i) {
+    if (bidx[i] < 0) continue;
+    const int64_t o = 5 + 4 * (int64_t)bidx[i];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) kahan(slab + o + c, slabc + o + c, (double)acc[i][c]);
+  }
+}
+
 template <typename R, int GEO, bool MIE, bool CB>
 __global__ __launch_bounds__(kBlock) void coop_kernel(const CoopParams<R>* __restrict__ pp) {
   const CoopParams<R>& p = *pp;
@@ -300,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void coop_kernel(const CoopParams<R>* __res
   __syncthreads();
 
   unsigned long long cnt[kCntN] = {};
-  double Aacc[5] = {0, 0, 0, 0, 0}, Acmp[5] = {0, 0, 0, 0, 0};
+  double Aacc[5] = {0, 0, 0, 0, 0};
   double* slab = p.slabs + (size_t)blockIdx.x * 2 * p.nout;
   double* slabc = slab + p.nout;
   const bool can_dt = (p.geo_kind == 1) || (p.nl == 1 && !p.isinf[0]);
@@ -596,65 +664,21 @@ __global__ __launch_bounds__(kBlock) void coop_kernel(const CoopParams<R>* __res
       const int per = (nb + kBlock - 1) / kBlock;
       const int flag = 1 << (p.range_kind[rg] == 1 ? 0 : 1);
       const int slot2 = p.range_kind[rg] == 1 ? 0 : 1;
-      for (int c0 = 0; c0 < per; c0 += kChunkMax) {
-        R bk[kChunkMax][7];
-        int bidx[kChunkMax];
-        R acc[kChunkMax][4];
-#pragma unroll
-        for (int i = 0; i < kChunkMax; ++i) {
-          const int lb = tid + kBlock * (c0 + i);
-          const bool valid = (c0 + i) < per && lb < nb;
-          const int j1 = jb + lb % nrow, j2 = lb / nrow;
-          const int ob = j1 + p.Nt * j2;
-          bidx[i] = valid ? ob : -1;
-          const R* src = p.rt_bin + 8 * (valid ? ob : 0);
-#pragma unroll
-          for (int c = 0; c < 7; ++c) bk[i][c] = src[c];
-          if (!valid) bk[i][5] = R(0);  // zero weight
-#pragma unroll
-          for (int c = 0; c < 4; ++c) acc[i][c] = R(0);
-        }
-        for (int e = 0; e < E; ++e) {
-          const R* rec = rtrec + kRtRec * e;
-          const int fl = (int)rec[15];
-          if (!(fl & flag)) continue;
-          R m[8], I[4], pos[3];
-#pragma unroll
-          for (int c = 0; c < 8; ++c) m[c] = rec[c];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) I[c] = rec[8 + c];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) pos[c] = rec[12 + c];
-          R part = R(0);
-#pragma unroll
-          for (int i = 0; i < kChunkMax; ++i) {
-            R o[4];
-            peel_I(sca, m, I, bk[i][0], bk[i][1], bk[i][2], bk[i][3], bk[i][4], o);
-            R t;
-            if (GEO == 0) {
-              t = bk[i][2] > R(0) ? -pos[0] * bk[i][6] : (pos[0] + pos[1]) * -bk[i][6];
-            } else {
-              R kx = bk[i][0] * pos[0] + bk[i][1] * pos[1] + bk[i][2] * pos[2];
-              R xx = pos[0] * pos[0] + pos[1] * pos[1] + pos[2] * pos[2];
-              t = -kx + M<R>::sqrt(kx * kx + p.tau[0] * p.tau[0] - xx);
-            }
-            R w = t <= p.tau0 ? bk[i][5] * M<R>::exp(-t) : R(0);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) acc[i][c] += o[c] * w;
-            part += o[0] * w;
-          }
-          part = wave_sum(part);
-          if (lane == 0) esc[(e * kWaves + wave) * 2 + slot2] += part;
-        }
-        // flush this chunk into the workgroup slab (bins are owned: no races)
-#pragma unroll
-        for (int i = 0; i < kChunkMax; ++i) {
-          if (bidx[i] < 0) continue;
-          const int64_t o = 5 + 4 * (int64_t)bidx[i];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) kahan(slab + o + c, slabc + o + c, (double)acc[i][c]);
-        }
+      // full register chunks, then the remainder in passes of 4, 2 and 1
+      int c0 = 0;
+      for (; c0 + kChunkMax <= per; c0 += kChunkMax)
+        rt_pass<kChunkMax, R, GEO>(p, sca, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
+                                   c0, flag, slot2);
+      if (kChunkMax > 4 && c0 + 4 <= per) {
+        rt_pass<4, R, GEO>(p, sca, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb, c0, flag, slot2);
+        c0 += 4;
       }
+      if (c0 + 2 <= per) {
+        rt_pass<2, R, GEO>(p, sca, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb, c0, flag, slot2);
+        c0 += 2;
+      }
+      if (c0 < per)
+        rt_pass<1, R, GEO>(p, sca, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb, c0, flag, slot2);
     }
 
     // ------------------------------------------ CB bins × events (cpu.jl:38-160)

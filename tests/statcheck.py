@@ -11,10 +11,13 @@ import math
 import numpy as np
 
 
-def compare(a, b, sigma, *, floor=0.0, label="", verbose=True):
+def compare(a, b, sigma, *, floor=0.0, dead_rel=1e-12, dof=None, label="", verbose=True):
     """z-statistics of a - b with per-element standard error sigma.
 
-    Elements with sigma <= floor must agree to within floor + 1e-12·scale.
+    Elements with sigma <= floor (deterministic values) must agree to within
+    floor + dead_rel·scale.  When sigma is itself estimated from dof degrees of
+    freedom (batch means), z follows Student's t: the exceedance probability
+    and E[z²] = dof/(dof-2) replace the Gaussian ones.
     Returns a dict of summary statistics and `ok`."""
     a = np.asarray(a, float).ravel()
     b = np.asarray(b, float).ravel()
@@ -26,11 +29,16 @@ def compare(a, b, sigma, *, floor=0.0, label="", verbose=True):
     dead_err = np.max(np.abs(a[~live] - b[~live])) if (~live).any() else 0.0
     n = int(live.sum())
     k = int((np.abs(z[live]) > 3).sum())
-    p = 0.0027
+    if dof:
+        from scipy.stats import t as student_t
+        p = float(2 * student_t.sf(3.0, dof))
+        ez2 = dof / (dof - 2.0)
+    else:
+        p, ez2 = 0.0027, 1.0
     kmax = n * p + 4 * math.sqrt(max(n * p * (1 - p), 1e-12)) + 3
     chi2 = float(np.mean(z[live] ** 2)) if n else 0.0
-    chi2max = 1 + 5 * math.sqrt(2.0 / max(n, 1)) + 0.05
-    ok = bool(k <= kmax and chi2 <= chi2max and dead_err <= floor + 1e-12 * scale)
+    chi2max = ez2 * (1 + 5 * math.sqrt(2.0 / max(n, 1))) + 0.05
+    ok = bool(k <= kmax and chi2 <= chi2max and dead_err <= floor + dead_rel * scale)
     out = dict(n=n, exceed3=k, exceed3_max=kmax, chi2dof=chi2, chi2dof_max=chi2max,
                max_abs_z=float(np.max(np.abs(z))) if n else 0.0, dead_err=float(dead_err), ok=ok)
     if verbose:
