@@ -151,7 +151,7 @@ extern "C" int mcrt_context_create(int device, mcrt_ctx** out) {
     return rc;
   }
   c->cu_count = prop.multiProcessorCount;
-  c->lds_per_block = prop.sharedMemPerBlock;
+  c->lds_per_block = prop.sharedMemPerBlock;  // 160 KiB on gfx950
   *out = c;
   return MCRT_OK;
 }
@@ -410,7 +410,7 @@ extern "C" int mcrt_plan_create(mcrt_ctx* ctx, const mcrt_config* cfg, const mcr
   if (precision == MCRT_PRECISION_FP64_PLAIN) {
     rc = create_plain(P, cfg, geom, I0, seed);
   } else {
-    rc = mcrt::coop_create(P->coop, ctx->stream, ctx->cu_count, cfg, geom, I0, seed,
+    rc = mcrt::coop_create(P->coop, ctx->stream, ctx->cu_count, ctx->lds_per_block, cfg, geom, I0, seed,
                            precision == MCRT_PRECISION_MIXED,
                            [P](size_t n, hipError_t* err) { return P->alloc(n, err); },
                            P->dA, P->dIrt, P->dIcb, P->dcnt, &ctx->err);

@@ -4,22 +4,24 @@
 // (cpu.jl:6-31) and collect_cb (cpu.jl:38-160); the work is laid out for
 // CDNA4 instead of one-ray-per-thread:
 //
-//  * Every thread of a 256-thread workgroup owns a ray slot.  One iteration
-//    advances every live slot by one scattering event (scatter + rejection +
-//    absorption), writes a compact event record to LDS (wave ballot +
-//    prefix sum over waves), refills terminated slots from the workgroup's
-//    ray range, and repeats until the range is exhausted.
+//  * A 512-thread workgroup (8 wave64, two per SIMD) owns SLOTS ray slots
+//    (256 in FP32, 128 in FP64).  One iteration advances every live slot by one
+//    scattering event (scatter + rejection + absorption), writes a compact
+//    event record to LDS (wave ballot + prefix over waves), refills terminated
+//    slots from the workgroup's ray range, and repeats until the range is
+//    exhausted.
 //  * The peel-off (8640 directions per event at 180×48 bins) and the CB bins
-//    are evaluated by the whole workgroup: each thread owns a fixed set of
-//    bins (register-resident direction constants), loops over the LDS event
-//    records (broadcast reads) and accumulates its bins in registers, so no
-//    two threads of a workgroup ever touch the same bin.  The escape sums
-//    Iref/Itrans that feed back into each ray's weight are wave-reduced per
-//    (event, bin chunk) in a fixed order.
+//    are evaluated by all 512 threads: each thread owns a fixed set of bins
+//    (register-resident direction constants), loops over the LDS event records
+//    (broadcast reads) and accumulates its bins in registers, so no two threads
+//    of a workgroup touch the same bin.  The escape sums Iref/Itrans that feed
+//    back into each ray's weight are wave-reduced per (event, bin chunk) and
+//    summed over waves in a fixed order.
 //  * Bins are flushed after each iteration into the workgroup's private FP64
 //    slab with Kahan compensation; one reduction kernel sums the slabs in
 //    workgroup order.  Results are bitwise reproducible for a fixed launch.
-//  * Mie F/csrn tables (and S when it fits) are staged in LDS.
+//  * Mie F/csrn tables (and S in FP32) are staged in LDS, padded by one zero
+//    node so that the interpolation needs no end-of-table branch.
 //  * CB uses O(1) path state per ray in FP64: the forward field of
 //    forward_E (utils.jl:199-216) advanced one step per event, and the
 //    reversed path of backward_E (utils.jl:223-241) as a normalised running
@@ -36,18 +38,31 @@
 
 namespace mcrt {
 
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
-constexpr int kRtRec = 16;  // R words of an RT event record
+constexpr int kThreads = 512;
+constexpr int kWaves = kThreads / 64;
+constexpr int kRtRec = 20;  // R words of an RT event record
 constexpr int kCbRec = 40;  // R words of a CB event record
 constexpr int kCbDx = 4;    // doubles of a CB event record (x_a - x_b, τab)
+
+// Per-precision launch shape: ray slots per workgroup and RT bins per
+// register chunk.
+template <typename R> struct Shape;
+template <> struct Shape<float> {
+  static constexpr int kSlots = 256;
+  static constexpr int kChunk = 4;
+  static constexpr bool kStageS = true;  // S table in LDS
+};
+template <> struct Shape<double> {
+  static constexpr int kSlots = 128;
+  static constexpr int kChunk = 2;
+  static constexpr bool kStageS = false;
+};
 
 template <typename R> struct CoopParams {
   int64_t N;
   int64_t maxscat;
   R min_int;
   R tau0;
-  double tau0d;
   int compute_cb;
   int geo_kind;  // 0 plane, 1 sphere
   int nl;
@@ -57,10 +72,9 @@ template <typename R> struct CoopParams {
   double tau_d[kMaxLayers];
   double kl1;
   // the one scatterer of the solve, in R (transport, bins) and FP64 (CB path state)
-  ScaView<R> sca;
+  ScaView<R> sca;        // F and S padded with one zero node
   ScaView<double> sca64;
-  int stage_F;  // F and csrn are staged in LDS
-  int stage_S;  // S is staged in LDS
+  R scale;               // (Nθ_table - 1)/π
   // RT bins: ranges of θ rows, [lower (transmission), upper (reflection)]
   int Nt, Np, Ntb, Npb;
   int nranges;
@@ -70,17 +84,13 @@ template <typename R> struct CoopParams {
   const double* cb_kik;  // [Ntb*Npb][4]: k_i + k_f (FP64)
   R I0[4];
   R k0[3];
-  double k0d[3];
   double f0[4];   // raw incidence frame (cosθ₀, sinθ₀, cosϕ₀, sinϕ₀)
   double Ei[4];   // E_from_I(I₀): (E1 re, E1 im, E2 re, E2 im)
   double norm0;   // |E_i|
   double kf0[3];  // -k₀ (exact backscatter)
-  double kfs[4];  // frame of -k₀
   uint64_t seed;
   uint32_t instance;
-  int chunk;      // RT bins per register chunk (≤ kChunkMax)
-  void* slot_mem;         // [blocks][kBlock] Slot<R>: per-slot ray state
-  // outputs
+  void* slot_mem;         // [blocks][SLOTS] Slot<R>: per-slot ray state
   double* slabs;          // [blocks][2][nout]: Kahan sum, compensation
   int64_t nout;           // 5 + nrt + ncb
   int64_t nrt, ncb;
@@ -88,7 +98,6 @@ template <typename R> struct CoopParams {
 };
 
 namespace {
-
 
 template <typename R> MCRT_DEV R wave_sum(R v) {
 #pragma unroll
@@ -102,6 +111,64 @@ MCRT_DEV void kahan(double* s, double* c, double x) {
   double t = *s + y;
   *c = (t - *s) - y;
   *s = t;
+}
+
+// arccos.  FP32: Abramowitz & Stegun 4.4.46 (Hastings), |error| ≤ 2e-8 on
+// [0, 1], with acos(-x) = π - acos(x).  FP64: the library acos.
+MCRT_DEV double acos_t(double x) { return ::acos(x); }
+MCRT_DEV float acos_t(float x) {
+  float a = __builtin_fabsf(x);
+  float p = -0.0012624911f;
+  p = __builtin_fmaf(p, a, 0.0066700901f);
+  p = __builtin_fmaf(p, a, -0.0170881256f);
+  p = __builtin_fmaf(p, a, 0.0308918810f);
+  p = __builtin_fmaf(p, a, -0.0501743046f);
+  p = __builtin_fmaf(p, a, 0.0889789874f);
+  p = __builtin_fmaf(p, a, -0.2145988016f);
+  p = __builtin_fmaf(p, a, 1.5707963050f);
+  float r = __builtin_amdgcn_sqrtf(1.0f - a) * p;
+  return x < 0.0f ? 3.14159265358979f - r : r;
+}
+
+// Phase matrix (P11, P12, P33, P34) at cosθ = mu: Rayleigh (rayleigh.jl:12-17)
+// or a lookup in the padded LDS table F (mie.jl:60-80, linear in θ).
+template <typename R, bool MIE>
+MCRT_DEV void phase_tab(const R* F, R scale, int imax, R mu, R P[4]) {
+  if constexpr (!MIE) {
+    R m2 = mu * mu;
+    P[0] = R(0.75) * (R(1) + m2);
+    P[1] = R(-0.75) * (R(1) - m2);
+    P[2] = R(1.5) * mu;
+    P[3] = R(0);
+  } else {
+    R sn = acos_t(clamp1(mu)) * scale;
+    int i0 = (int)sn;
+    i0 = i0 > imax ? imax : i0;
+    R w1 = sn - R(i0);
+    R w0 = R(i0 + 1) - sn;
+    const R* a = F + 4 * i0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) P[c] = w0 * a[c] + w1 * a[4 + c];
+  }
+}
+
+// Amplitude matrix diag(S11, S22) (rayleigh.jl:20-23; mie.jl:83-99 with its
+// weights (msn+1-sn, sn-msn): the zero pad node reproduces the skipped term).
+template <typename R, bool MIE>
+MCRT_DEV void amp_tab(const R* S, R scale, int imax, R mu, Cx<R>& S11, Cx<R>& S22) {
+  if constexpr (!MIE) {
+    S11 = {R(0), R(1)};
+    S22 = {mu * R(0), mu};
+  } else {
+    R sn = acos_t(clamp1(mu)) * scale;
+    int i0 = (int)sn;
+    i0 = i0 > imax ? imax : i0;
+    R w0 = R(i0 + 2) - sn;
+    R w1 = sn - R(i0 + 1);
+    const R* a = S + 4 * i0;
+    S11 = {w0 * a[0] + w1 * a[4], w0 * a[1] + w1 * a[5]};
+    S22 = {w0 * a[2] + w1 * a[6], w0 * a[3] + w1 * a[7]};
+  }
 }
 
 // ------------------------------------------------------------- geometry (R)
@@ -139,7 +206,8 @@ template <typename R> MCRT_DEV void frame_matrix(Fr<R> f, R m[8]) {
 
 // The frame part shared by update_I and update_E (utils.jl:92-105) for a
 // direction k with basis vector Eh (from get_basis(k)) and an incident frame
-// matrix m: returns cosθₖ (= μ), cos/sin ϕₖ, and cosψ, Eh·Evk.
+// matrix m: returns μ = cosθₖ, cos/sin ϕₖ, cosψ and Eh·Evk.  Branch-free: the
+// polar case of spherical_from_cartesian (utils.jl:12-14) is a select.
 template <typename R>
 MCRT_DEV void frame_part(const R* m, R kx, R ky, R kz, R eh1, R eh2, R& mu, R& cpk, R& spk,
                          R& cpsi, R& ehev) {
@@ -147,16 +215,12 @@ MCRT_DEV void frame_part(const R* m, R kx, R ky, R kz, R eh1, R eh2, R& mu, R& c
   R ry = m[1] * kx + m[4] * ky;
   R rz = m[2] * kx + m[5] * ky + m[7] * kz;
   R s2 = rx * rx + ry * ry;
-  R ctk, stk;
-  if (s2 < R(1e-24)) {
-    ctk = jl_sign(rz); stk = R(0); cpk = R(1); spk = R(0);
-  } else {
-    ctk = rz;
-    stk = sqrt0(R(1) - rz * rz);
-    R inv = M<R>::rsqrt(s2);
-    cpk = rx * inv;
-    spk = ry * inv;
-  }
+  const bool polar = s2 < R(1e-24);
+  R inv = M<R>::rsqrt(polar ? R(1) : s2);
+  cpk = polar ? R(1) : rx * inv;
+  spk = polar ? R(0) : ry * inv;
+  R ctk = polar ? jl_sign(rz) : rz;
+  R stk = polar ? R(0) : sqrt0(R(1) - rz * rz);
   mu = rz;
   R ehk1 = m[0] * spk - m[1] * cpk;
   R ehk2 = m[3] * spk - m[4] * cpk;
@@ -167,16 +231,16 @@ MCRT_DEV void frame_part(const R* m, R kx, R ky, R kz, R eh1, R eh2, R& mu, R& c
   ehev = eh1 * evk1 + eh2 * evk2;
 }
 
-// Peel-off Stokes vector of one direction before attenuation:
-// update_I (utils.jl:91-112) with the basis of the bin precomputed.
-template <typename R>
-MCRT_DEV void peel_I(const ScaView<R>& s, const R* m, const R* I, R kx, R ky, R kz, R eh1, R eh2,
-                     R out[4]) {
+// Peel-off Stokes vector of one direction before attenuation: update_I
+// (utils.jl:91-112) with the bin's basis precomputed.
+template <typename R, bool MIE>
+MCRT_DEV void peel_I(const R* F, R scale, int imax, const R* m, const R* I, R kx, R ky, R kz,
+                     R eh1, R eh2, R out[4]) {
   R mu, cpk, spk, cpsi, ehev;
   frame_part(m, kx, ky, kz, eh1, eh2, mu, cpk, spk, cpsi, ehev);
   R spsi = -ehev;
   R P[4];
-  phase_eval(s, mu, P);
+  phase_tab<R, MIE>(F, scale, imax, mu, P);
   R c2 = cpk * cpk - spk * spk, s2 = R(-2) * spk * cpk;
   R i1 = c2 * I[1] + s2 * I[2];
   R i2 = -s2 * I[1] + c2 * I[2];
@@ -192,13 +256,13 @@ MCRT_DEV void peel_I(const ScaView<R>& s, const R* m, const R* I, R kx, R ky, R 
 }
 
 // update_E (utils.jl:170-192) with the basis of kf precomputed.
-template <typename R>
-MCRT_DEV void peel_E(const ScaView<R>& s, const R* m, Cx<R>& E1, Cx<R>& E2, R kx, R ky, R kz,
-                     R eh1, R eh2) {
+template <typename R, bool MIE>
+MCRT_DEV void peel_E(const R* S, R scale, int imax, const R* m, Cx<R>& E1, Cx<R>& E2, R kx, R ky,
+                     R kz, R eh1, R eh2) {
   R mu, cpk, spk, cpsi, spsi;
   frame_part(m, kx, ky, kz, eh1, eh2, mu, cpk, spk, cpsi, spsi);
   Cx<R> S11, S22;
-  amplitude(s, mu, S11, S22);
+  amp_tab<R, MIE>(S, scale, imax, mu, S11, S22);
   Cx<R> e1 = E1 * cpk + E2 * spk;
   Cx<R> e2 = (-E1) * spk + E2 * cpk;
   e1 = S11 * e1;
@@ -217,7 +281,6 @@ MCRT_DEV void phase_cs(double phi, float& c, float& s) {
   c = __builtin_amdgcn_cosf(f);
 }
 
-// 2×2 complex FP64 products for the reversed-path state.
 MCRT_DEV void lmap_normalise(Lmap<double>& Q, double& lq) {
   double m = fmax(fmax(fmax(fabs(Q.a.re), fabs(Q.a.im)), fmax(fabs(Q.b.re), fabs(Q.b.im))),
                   fmax(fmax(fabs(Q.c.re), fabs(Q.c.im)), fmax(fabs(Q.d.re), fabs(Q.d.im))));
@@ -230,7 +293,7 @@ MCRT_DEV void lmap_normalise(Lmap<double>& Q, double& lq) {
   }
 }
 
-// Per-slot ray state (lives in registers).
+// Per-slot ray state (HBM between phases).
 template <typename R> struct Slot {
   int live;
   int idx, idxp, idx1;
@@ -252,18 +315,18 @@ template <typename R> struct Slot {
 
 // One pass of the peel-off (cpu.jl:11-28) over CH register-resident bins per
 // thread and all E events of the iteration.  Thread tid owns the bins
-// lb = tid + kBlock·(c0 + i) of the θ-range [jb, jb + nrow) × Nϕ; the escape
+// lb = tid + kThreads·(c0 + i) of the θ-range [jb, jb + nrow) × Nϕ; the escape
 // sum of each event is wave-reduced and added by lane 0 to esc in chunk order.
-template <int CH, typename R, int GEO>
-MCRT_DEV void rt_pass(const CoopParams<R>& p, const ScaView<R>& sca, const R* rtrec, R* esc,
-                      double* slab, double* slabc, int E, int tid, int lane, int wave, int jb,
-                      int nrow, int nb, int c0, int flag, int slot2) {
+template <int CH, typename R, int GEO, bool MIE>
+MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc, double* slab,
+                      double* slabc, int E, int tid, int lane, int wave, int jb, int nrow, int nb,
+                      int c0, int flag, int slot2) {
   R bk[CH][7];
   int bidx[CH];
   R acc[CH][4];
 #pragma unroll
   for (int i = 0; i < CH; ++i) {
-    const int lb = tid + kBlock * (c0 + i);
+    const int lb = tid + kThreads * (c0 + i);
     const bool valid = lb < nb;
     const int j1 = jb + lb % nrow, j2 = lb / nrow;
     const int ob = j1 + p.Nt * j2;
@@ -275,32 +338,33 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const ScaView<R>& sca, const R* rt
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[i][c] = R(0);
   }
-  const R tau2 = p.tau[0] * p.tau[0];
+  const R scale = p.scale;
+  const int imax = p.sca.Nt - 1;
+  const R tau0 = p.tau0;
   for (int e = 0; e < E; ++e) {
     const R* rec = rtrec + kRtRec * e;
-    const int fl = (int)rec[15];
+    const int fl = (int)rec[16];
     if (!(fl & flag)) continue;
-    R m[8], I[4], pos[3];
+    R m[8], I[4], pos[4];
 #pragma unroll
     for (int c = 0; c < 8; ++c) m[c] = rec[c];
 #pragma unroll
     for (int c = 0; c < 4; ++c) I[c] = rec[8 + c];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) pos[c] = rec[12 + c];
+    for (int c = 0; c < 4; ++c) pos[c] = rec[12 + c];
     R part = R(0);
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       R o[4];
-      peel_I(sca, m, I, bk[i][0], bk[i][1], bk[i][2], bk[i][3], bk[i][4], o);
+      peel_I<R, MIE>(F, scale, imax, m, I, bk[i][0], bk[i][1], bk[i][2], bk[i][3], bk[i][4], o);
       R t;
-      if (GEO == 0) {  // distance_to_boundary, plane.jl:97-105
-        t = bk[i][2] > R(0) ? -pos[0] * bk[i][6] : (pos[0] + pos[1]) * -bk[i][6];
-      } else {         // sphere.jl:69-73
+      if (GEO == 0) {  // distance_to_boundary, plane.jl:97-105 (pos[3]: τ of the layer or ∞)
+        t = bk[i][2] > R(0) ? -pos[2] * bk[i][6] : (pos[2] + pos[3]) * -bk[i][6];
+      } else {         // sphere.jl:69-73 (pos[3] = τ² - x·x)
         R kx = bk[i][0] * pos[0] + bk[i][1] * pos[1] + bk[i][2] * pos[2];
-        R xx = pos[0] * pos[0] + pos[1] * pos[1] + pos[2] * pos[2];
-        t = -kx + M<R>::sqrt(kx * kx + tau2 - xx);
+        t = -kx + sqrt0(kx * kx + pos[3]);
       }
-      R w = t <= p.tau0 ? bk[i][5] * M<R>::exp(-t) : R(0);
+      R w = t <= tau0 ? bk[i][5] * M<R>::exp(-t) : R(0);
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[i][c] += o[c] * w;
       part += o[0] * w;
@@ -318,53 +382,126 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const ScaView<R>& sca, const R* rt
   }
 }
 
+// CB bins × events (cpu.jl:83-158): thread tid owns CB bins tid, tid + kThreads, ….
+template <typename R, int GEO, bool MIE>
+MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* cbrec,
+                      const double* cbdx, double* slab, double* slabc, int E, int tid) {
+  const int nb = p.Ntb * p.Npb;
+  const R scale = p.scale;
+  const int imax = p.sca.Nt - 1;
+  const R tau0 = p.tau0;
+  const R tau2 = p.tau[0] * p.tau[0];
+  for (int lb = tid; lb < nb; lb += kThreads) {
+    const R* bk = p.cb_bin + 8 * lb;
+    const R kfx = bk[0], kfy = bk[1], kfz = bk[2], eh1 = bk[3], eh2 = bk[4];
+    const R c2p = bk[5], s2p = bk[6], ikz = bk[7];
+    const double* kk = p.cb_kik + 4 * lb;
+    const double kikx = kk[0], kiky = kk[1], kikz = kk[2];
+    R acc[4] = {R(0), R(0), R(0), R(0)};
+    for (int e = 0; e < E; ++e) {
+      const R* cr = cbrec + kCbRec * e;
+      const int kind = (int)cr[39];
+      if (kind == 0) continue;
+      const V3<R> xa = {cr[24], cr[25], cr[26]};
+      R ta;
+      if (GEO == 0) ta = kfz > R(0) ? -xa.z * ikz : (xa.z + cr[30]) * -ikz;
+      else {
+        R kx = kfx * xa.x + kfy * xa.y + kfz * xa.z;
+        ta = -kx + sqrt0(kx * kx + tau2 - dot(xa, xa));
+      }
+      if (!(ta <= tau0)) continue;
+      R o[4];
+      if (kind == 1) {  // n = 1 (cpu.jl:131-158)
+        peel_I<R, MIE>(F, scale, imax, cr + 8, cr, kfx, kfy, kfz, eh1, eh2, o);
+        R na = M<R>::exp(-ta);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) o[c] = o[c] * na;
+      } else {          // n > 1 (cpu.jl:83-127)
+        const V3<R> xb = {cr[27], cr[28], cr[29]};
+        Cx<R> a1 = {cr[0], cr[1]}, a2 = {cr[2], cr[3]};
+        Cx<R> b1 = {cr[4], cr[5]}, b2 = {cr[6], cr[7]};
+        peel_E<R, MIE>(S, scale, imax, cr + 8, a1, a2, kfx, kfy, kfz, eh1, eh2);
+        peel_E<R, MIE>(S, scale, imax, cr + 16, b1, b2, kfx, kfy, kfz, eh1, eh2);
+        const double* dx = cbdx + kCbDx * e;
+        const double dphi = p.kl1 * (kikx * dx[0] + kiky * dx[1] + kikz * dx[2]);
+        const R tab = (R)dx[3];
+        R na = M<R>::exp(R(-0.5) * ta);
+        a1 = a1 * na; a2 = a2 * na;
+        R Ia[4];
+        I_from_E(a1, a2, Ia);
+        R tb;
+        if (GEO == 0) tb = kfz > R(0) ? -xb.z * ikz : (xb.z + cr[31]) * -ikz;
+        else {
+          R kx = kfx * xb.x + kfy * xb.y + kfz * xb.z;
+          tb = -kx + sqrt0(kx * kx + tau2 - dot(xb, xb));
+        }
+        const bool bok = M<R>::fabs(tb + tab) <= tau0;
+        R nb_ = bok ? M<R>::exp(R(-0.5) * (tb + tab)) : R(0);
+        b1 = b1 * nb_; b2 = b2 * nb_;
+        R Ib[4];
+        I_from_E(b1, b2, Ib);
+        R cph, sph_;
+        phase_cs(dphi, cph, sph_);
+        Cx<R> ep = {cph, sph_};
+        Cx<R> e1 = a1 * ep + b1, e2 = a2 * ep + b2;
+        I_from_E(e1, e2, o);
+        R rr = M<R>::div(Ia[0], Ia[0] + Ib[0]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) o[c] = o[c] * rr;
+      }
+      rotate(o, c2p, s2p);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] += o[c];
+    }
+    const int64_t ob = 5 + p.nrt + 4 * (int64_t)lb;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) kahan(slab + ob + c, slabc + ob + c, (double)acc[c]);
+  }
+}
+
 template <typename R, int GEO, bool MIE, bool CB>
-__global__ __launch_bounds__(kBlock) void coop_kernel(const CoopParams<R>* __restrict__ pp) {
+__global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __restrict__ pp) {
+  constexpr int SLOTS = Shape<R>::kSlots;
+  constexpr int SW = SLOTS / 64;  // waves owning slots
+  constexpr int CH = Shape<R>::kChunk;
+  constexpr bool STAGE_S = Shape<R>::kStageS && MIE && CB;
   const CoopParams<R>& p = *pp;
-  // RT bins per register chunk: 8 in FP32, 4 in FP64
-  constexpr int kChunkMax = sizeof(R) == 4 ? 8 : 4;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
+  const bool owner = tid < SLOTS;
 
   // ---------------------------------------------------------- LDS layout
-  // (all offsets are multiples of 16 bytes)
+  // (every block is a multiple of 16 bytes; must match lds_plan)
   unsigned char* q = lds_raw;
   auto take = [&q](size_t bytes) { unsigned char* r = q; q += (bytes + 15) & ~size_t(15); return r; };
-  R* ldsF = nullptr;
-  R* ldsS = nullptr;
-  R* ldsC = nullptr;
-  if (MIE && p.stage_F) {
-    ldsF = (R*)take(sizeof(R) * 4 * p.sca.Nt);
-    ldsC = (R*)take(sizeof(R) * (p.sca.rp + 1));
-  }
-  if (MIE && CB && p.stage_S) ldsS = (R*)take(sizeof(R) * 4 * p.sca.Nt);
-  R* rtrec = (R*)take(sizeof(R) * kRtRec * kBlock);
-  R* cbrec = CB ? (R*)take(sizeof(R) * kCbRec * kBlock) : nullptr;
-  double* cbdx = CB ? (double*)take(sizeof(double) * kCbDx * kBlock) : nullptr;
-  R* esc = (R*)take(sizeof(R) * kBlock * kWaves * 2);
-  int* wcnt = (int*)take(sizeof(int) * 16);
-  int* misc = (int*)take(sizeof(int) * 8);
+  R* rtrec = (R*)take(sizeof(R) * kRtRec * SLOTS);
+  R* cbrec = CB ? (R*)take(sizeof(R) * kCbRec * SLOTS) : nullptr;
+  double* cbdx = CB ? (double*)take(sizeof(double) * kCbDx * SLOTS) : nullptr;
+  R* esc = (R*)take(sizeof(R) * SLOTS * kWaves * 2);
+  int* wcnt = (int*)take(sizeof(int) * 32);
+  R* ldsF = MIE ? (R*)take(sizeof(R) * 4 * (p.sca.Nt + 1)) : nullptr;
+  R* ldsC = MIE ? (R*)take(sizeof(R) * (p.sca.rp + 1)) : nullptr;
+  R* ldsS = STAGE_S ? (R*)take(sizeof(R) * 4 * (p.sca.Nt + 1)) : nullptr;
 
-  ScaView<R> sca = p.sca;
-  if (MIE && p.stage_F) {
-    for (int i = tid; i < 4 * p.sca.Nt; i += kBlock) ldsF[i] = p.sca.F[i];
-    for (int i = tid; i <= p.sca.rp; i += kBlock) ldsC[i] = p.sca.csrn[i];
+  ScaView<R> sca = p.sca;  // transport view (F/csrn in LDS)
+  if constexpr (MIE) {
+    for (int i = tid; i < 4 * (p.sca.Nt + 1); i += kThreads) ldsF[i] = p.sca.F[i];
+    for (int i = tid; i <= p.sca.rp; i += kThreads) ldsC[i] = p.sca.csrn[i];
     sca.F = ldsF;
     sca.csrn = ldsC;
   }
-  if (MIE && CB && p.stage_S) {
-    for (int i = tid; i < 4 * p.sca.Nt; i += kBlock) ldsS[i] = p.sca.S[i];
-    sca.S = ldsS;
+  if constexpr (STAGE_S) {
+    for (int i = tid; i < 4 * (p.sca.Nt + 1); i += kThreads) ldsS[i] = p.sca.S[i];
   }
+  const R* Sbins = STAGE_S ? (const R*)ldsS : p.sca.S;
   const Geo<R> geo{&p};
 
   // ------------------------------------------------------ ray range of WG
   const int64_t G = gridDim.x;
   const int64_t rb = p.N * (int64_t)blockIdx.x / G;
   const int64_t re = p.N * ((int64_t)blockIdx.x + 1) / G;
-  if (tid == 0) { misc[0] = 0; }
   __syncthreads();
 
   unsigned long long cnt[kCntN] = {};
@@ -375,402 +512,328 @@ __global__ __launch_bounds__(kBlock) void coop_kernel(const CoopParams<R>* __res
 
   // Per-slot ray state lives in HBM between the transport and update phases
   // so that the bin loops run with a small register footprint.
-  Slot<R>* const myslot = (Slot<R>*)p.slot_mem + (size_t)blockIdx.x * kBlock + tid;
+  Slot<R>* const myslot = (Slot<R>*)p.slot_mem + (size_t)blockIdx.x * SLOTS + (owner ? tid : 0);
   int live = 0;           // this slot holds a ray
   int64_t next_ray = rb;  // identical in every thread
   int e_me = 0;
 
   for (;;) {
     {
-    Slot<R> s;
-    if (live) s = *myslot;
-    s.live = live;
-    // --------------------------------------- refill free slots (in order)
-    {
-      unsigned long long need = __ballot(!s.live);
-      if (lane == 0) wcnt[wave] = __popcll(need);
-      __syncthreads();
-      int off = 0;
-      for (int w = 0; w < wave; ++w) off += wcnt[w];
-      int tot = 0;
-      for (int w = 0; w < kWaves; ++w) tot += wcnt[w];
-      off += __popcll(need & ((1ull << lane) - 1ull));
-      __syncthreads();
-      if (!s.live && next_ray + off < re) {
-        // start a new ray: initial_x, direct transmission, initial_propagation
-        s.live = 1;
-        s.ray = (uint64_t)(next_ray + off);
-        s.draw = 0;
-        Rng rng{p.seed, s.ray, p.instance, 0};
-        for (int c = 0; c < 4; ++c) s.I[c] = p.I0[c];
-        s.k = {p.k0[0], p.k0[1], p.k0[2]};
-        if (GEO == 0) {
-          s.x = {R(0), R(0), R(0)};
-        } else {  // sphere.jl:43-47
-          R tau = p.tau[0];
-          R r = tau * M<R>::sqrt(rng.template u<R>());
-          R ph = M<R>::two_pi * rng.template u<R>();
-          R sp_, cp_;
-          M<R>::sincos(ph, &sp_, &cp_);
-          s.x = {r * cp_, r * sp_, M<R>::sqrt(tau * tau - r * r)};
+      Slot<R> s;
+      if (live) s = *myslot;
+      s.live = live;
+      // ------------------------------------- refill free slots (in order)
+      {
+        unsigned long long need = __ballot(owner && !s.live);
+        if (lane == 0) wcnt[wave] = __popcll(need);
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int w = 0; w < SW; ++w) {
+          if (w < wave) off += wcnt[w];
+          tot += wcnt[w];
         }
-        s.idx = 0;
-        s.n = 0;
-        if (can_dt) {  // cpu.jl:175-180
-          R tp = geo.dist(s.x, s.k, 0);
-          R dt = tp <= p.tau0 ? M<R>::exp(-tp) : R(0);
-          Aacc[0] += (double)(s.I[0] * dt);
-          for (int c = 0; c < 4; ++c) s.I[c] = s.I[c] * (R(1) - dt);
-        }
-        {  // initial_propagation (plane.jl:77-95, sphere.jl:54-67)
-          R t;
+        off += __popcll(need & ((1ull << lane) - 1ull));
+        if (owner && !s.live && next_ray + off < re) {
+          // a new ray: initial_x, direct transmission, initial_propagation
+          s.live = 1;
+          s.ray = (uint64_t)(next_ray + off);
+          Rng rng{p.seed, s.ray, p.instance, 0};
+#pragma unroll
+          for (int c = 0; c < 4; ++c) s.I[c] = p.I0[c];
+          s.k = {p.k0[0], p.k0[1], p.k0[2]};
           if (GEO == 0) {
-            if (p.isinf[0]) {
-              t = -M<R>::log(rng.template u<R>());
+            s.x = {R(0), R(0), R(0)};
+          } else {  // sphere.jl:43-47
+            R tau = p.tau[0];
+            R r = tau * M<R>::sqrt(rng.template u<R>());
+            R ph = M<R>::two_pi * rng.template u<R>();
+            R sp_, cp_;
+            M<R>::sincos(ph, &sp_, &cp_);
+            s.x = {r * cp_, r * sp_, M<R>::sqrt(tau * tau - r * r)};
+          }
+          s.idx = 0;
+          s.n = 0;
+          if (can_dt) {  // cpu.jl:175-180
+            R tp = geo.dist(s.x, s.k, 0);
+            R dt = tp <= p.tau0 ? M<R>::exp(-tp) : R(0);
+            Aacc[0] += (double)(s.I[0] * dt);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s.I[c] = s.I[c] * (R(1) - dt);
+          }
+          {  // initial_propagation (plane.jl:77-95, sphere.jl:54-67)
+            R t;
+            if (GEO == 0) {
+              if (p.isinf[0]) {
+                t = -M<R>::log(rng.template u<R>());
+              } else {
+                R tp = geo.dist(s.x, s.k, 0);
+                if (can_dt && tp <= p.tau0) {
+                  t = -M<R>::log(R(1) - rng.template u<R>() * (R(1) - M<R>::exp(-tp)));
+                } else {
+                  t = tp + R(1);
+                  for (int it = 0; t > tp && it < (1 << 24); ++it) t = -M<R>::log(rng.template u<R>());
+                }
+              }
             } else {
-              R tp = geo.dist(s.x, s.k, 0);
-              if (can_dt && tp <= p.tau0) {
+              R tp = R(2) * s.x.z;
+              if (tp <= p.tau0) {
                 t = -M<R>::log(R(1) - rng.template u<R>() * (R(1) - M<R>::exp(-tp)));
               } else {
                 t = tp + R(1);
-                for (int it = 0; t > tp && it < (1 << 24); ++it) t = -M<R>::log(rng.template u<R>());
+                for (int it = 0; t > tp && it < (1 << 24); ++it)
+                  t = -M<R>::log(R(1) - rng.template u<R>());
               }
             }
-          } else {
-            R tp = R(2) * s.x.z;
-            if (tp <= p.tau0) {
-              t = -M<R>::log(R(1) - rng.template u<R>() * (R(1) - M<R>::exp(-tp)));
-            } else {
-              t = tp + R(1);
-              for (int it = 0; t > tp && it < (1 << 24); ++it)
-                t = -M<R>::log(R(1) - rng.template u<R>());
-            }
+            s.x = {s.x.x + t * s.k.x, s.x.y + t * s.k.y, s.x.z + t * s.k.z};
           }
-          s.x = {s.x.x + t * s.k.x, s.x.y + t * s.k.y, s.x.z + t * s.k.z};
+          s.draw = rng.draw;
+          ++cnt[kCntRays];
+          if (!(s.I[0] >= p.min_int && p.maxscat > 0)) {  // the loop of cpu.jl:186 never runs
+            if (!isfinite((double)s.I[0])) ++cnt[kCntNonfinite];
+            Aacc[4] += (double)s.I[0];
+            s.live = 0;
+          }
+          if (CB) {
+            s.Ea1 = {p.Ei[0], p.Ei[1]};
+            s.Ea2 = {p.Ei[2], p.Ei[3]};
+            s.enorma = 0.0;
+            s.ksf = {p.f0[0], p.f0[1], p.f0[2], p.f0[3]};
+            s.lq = 0.0;
+          }
+        }
+        next_ray += tot;
+      }
+
+      // --------------------------------- compaction of the live slots
+      unsigned long long lv = __ballot(owner && s.live);
+      if (lane == 0) wcnt[16 + wave] = __popcll(lv);
+      __syncthreads();
+      int E = 0, eoff = 0;
+      for (int w = 0; w < SW; ++w) {
+        if (w < wave) eoff += wcnt[16 + w];
+        E += wcnt[16 + w];
+      }
+      if (E == 0) break;
+      e_me = eoff + __popcll(lv & ((1ull << lane) - 1ull));
+
+      // ------------------------------------ transport: one event per slot
+      if (owner && s.live) {
+        Rng rng{p.seed, s.ray, p.instance, s.draw};
+        s.n += 1;
+        const int64_t n = s.n;
+        if (n == 1) {
+          s.x1d = {(double)s.x.x, (double)s.x.y, (double)s.x.z};
+          s.idx1 = s.idx;
+        }
+        V3<double> kd = {(double)s.k.x, (double)s.k.y, (double)s.k.z};
+        if (CB) {
+          // forward_E advanced by one step (utils.jl:205-213)
+          if (n >= 2) {
+            update_E(p.sca64, s.Ea1, s.Ea2, kd, s.ksf);
+            double a = hypot(s.Ea1.re, s.Ea1.im), b = hypot(s.Ea2.re, s.Ea2.im);
+            double norm2 = sqrt(a * a + b * b);
+            double r = p.norm0 / norm2;
+            s.Ea1 = s.Ea1 * r;
+            s.Ea2 = s.Ea2 * r;
+            s.enorma += log(p.norm0 / norm2);
+            s.ksf = sph(kd);
+          }
+          // reversed path as a running product (utils.jl:230-238)
+          if (n == 2) {
+            s.Q = {{1, 0}, {0, 0}, {0, 0}, {1, 0}};
+            s.lq = 0.0;
+            s.kbs = sph(-kd);
+          } else if (n >= 3) {
+            Lmap<double> L = update_E_map(p.sca64, -s.kprev, sph(-kd));
+            s.Q = mul(s.Q, L);
+            lmap_normalise(s.Q, s.lq);
+          }
+          s.kprev = kd;
+        }
+        // scatter + free path with rejection (utils.jl:294-323, cpu.jl:194-235)
+        {
+          Fr<R> ks = sph(s.k);
+          for (int it = 0;; ++it) {
+            R ct = phase_sample(sca, rng.template u<R>());
+            R P[4];
+            phase_eval(sca, ct, P);
+            R eq = -P[1] * s.I[1] / (P[0] * s.I[0]);
+            R eu = -P[1] * s.I[2] / (P[0] * s.I[0]);
+            R ee = M<R>::sqrt(eq * eq + eu * eu);
+            R phi;
+            if (ee > R(1e-12)) {
+              R gam = M<R>::acos(clamp1(eq / ee));
+              if (eu < R(0)) gam = M<R>::two_pi - gam;
+              R eec = ee < R(1) ? ee : R(1);
+              R ma = M<R>::four_pi * rng.template u<R>() + gam - eec * M<R>::sin(gam);
+              R ea = kepler(ma, eec);
+              phi = R(0.5) * (ea - gam);
+            } else {
+              phi = M<R>::two_pi * rng.template u<R>();
+            }
+            R st = sqrt0(R(1) - ct * ct);
+            R sph_, cph_;
+            M<R>::sincos(phi, &sph_, &cph_);
+            s.kp = ray_to_norm(V3<R>{st * cph_, st * sph_, ct}, ks);
+            R Ipt[4];
+            update_I(sca, s.I, s.kp, ks, Ipt);
+            R rr = s.I[0] / Ipt[0];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s.Ip[c] = Ipt[c] * rr;
+            R t = -M<R>::log(rng.template u<R>());
+            s.xp = {s.x.x + t * s.kp.x, s.x.y + t * s.kp.y, s.x.z + t * s.kp.z};
+            int pb;
+            if (GEO == 0)
+              pb = s.xp.z > R(0) ? -1 : ((!p.isinf[s.idx] && s.xp.z < -p.tau[s.idx]) ? 1 : 0);
+            else
+              pb = M<R>::sqrt(dot(s.xp, s.xp)) > p.tau[0] ? -2 : 0;
+            s.idxp = s.idx + pb;
+            bool ok;
+            if (pb == -2) ok = false;
+            else if (pb == -1) {
+              ok = s.idxp >= 0;
+              if (ok) s.xp = {s.x.x, s.x.y, -p.tau[s.idx]};
+            } else if (pb == 1) {
+              ok = s.idxp <= p.nl - 1;
+              if (ok) s.xp = {s.x.x, s.x.y, R(0)};
+            } else ok = true;
+            if (ok) break;
+            ++cnt[kCntRej];
+            if (it >= (1 << 24)) { s.idxp = s.idx; s.xp = s.x; break; }
+          }
         }
         s.draw = rng.draw;
-        ++cnt[kCntRays];
-        if (!(s.I[0] >= p.min_int && p.maxscat > 0)) {  // the loop of cpu.jl:186 never runs
-          if (!isfinite((double)s.I[0])) ++cnt[kCntNonfinite];
-          Aacc[4] += (double)s.I[0];
-          s.live = 0;
+        // absorption (cpu.jl:238-239)
+        R om = p.omega[s.idx];
+        Aacc[1] += (double)((R(1) - om) * s.I[0]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s.I[c] = s.I[c] * om;
+        // near flags (cpu.jl:245-246)
+        R dfirst, dlast;
+        if (GEO == 0) {
+          dfirst = s.idx == 0 ? M<R>::fabs(s.x.z) : R(INFINITY);
+          dlast = (s.idx == p.nl - 1 && !p.isinf[s.idx]) ? p.tau[s.idx] - M<R>::fabs(s.x.z) : R(INFINITY);
+        } else {
+          dfirst = p.tau[0] - M<R>::sqrt(dot(s.x, s.x));
+          dlast = R(INFINITY);
+        }
+        const int near_up = dfirst <= p.tau0;
+        const int near_lo = dlast <= p.tau0;
+        // RT event record: frame matrix, I, position, flags
+        R* rec = rtrec + kRtRec * e_me;
+        frame_matrix(sph(s.k), rec);
+        rec[8] = s.I[0]; rec[9] = s.I[1]; rec[10] = s.I[2]; rec[11] = s.I[3];
+        rec[12] = s.x.x; rec[13] = s.x.y; rec[14] = s.x.z;
+        if (GEO == 0) rec[15] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
+        else rec[15] = p.tau[0] * p.tau[0] - dot(s.x, s.x);
+        rec[16] = R(near_up | (near_lo << 1));
+        for (int w = 0; w < kWaves; ++w) {
+          esc[(e_me * kWaves + w) * 2] = R(0);
+          esc[(e_me * kWaves + w) * 2 + 1] = R(0);
         }
         if (CB) {
-          s.Ea1 = {p.Ei[0], p.Ei[1]};
-          s.Ea2 = {p.Ei[2], p.Ei[3]};
-          s.enorma = 0.0;
-          s.ksf = {p.f0[0], p.f0[1], p.f0[2], p.f0[3]};
-          s.lq = 0.0;
-        }
-      }
-      next_ray += tot;
-    }
-
-    // --------------------------------- compaction of the live slots
-    unsigned long long lv = __ballot(s.live);
-    if (lane == 0) wcnt[8 + wave] = __popcll(lv);
-    __syncthreads();
-    int E = 0, eoff = 0;
-    for (int w = 0; w < kWaves; ++w) {
-      if (w < wave) eoff += wcnt[8 + w];
-      E += wcnt[8 + w];
-    }
-    if (E == 0) break;
-    e_me = eoff + __popcll(lv & ((1ull << lane) - 1ull));
-
-    // ------------------------------------ transport: one event per slot
-    if (s.live) {
-      Rng rng{p.seed, s.ray, p.instance, s.draw};
-      s.n += 1;
-      const int64_t n = s.n;
-      if (n == 1) {
-        s.x1d = {(double)s.x.x, (double)s.x.y, (double)s.x.z};
-        s.idx1 = s.idx;
-      }
-      V3<double> kd = {(double)s.k.x, (double)s.k.y, (double)s.k.z};
-      if (CB) {
-        // forward_E advanced by one step (utils.jl:205-213)
-        if (n >= 2) {
-          update_E(p.sca64, s.Ea1, s.Ea2, kd, s.ksf);
-          double a = hypot(s.Ea1.re, s.Ea1.im), b = hypot(s.Ea2.re, s.Ea2.im);
-          double norm2 = sqrt(a * a + b * b);
-          double r = p.norm0 / norm2;
-          s.Ea1 = s.Ea1 * r;
-          s.Ea2 = s.Ea2 * r;
-          s.enorma += log(p.norm0 / norm2);
-          s.ksf = sph(kd);
-        }
-        // reversed path as a running product (utils.jl:230-238)
-        if (n == 2) {
-          s.Q = {{1, 0}, {0, 0}, {0, 0}, {1, 0}};
-          s.lq = 0.0;
-          s.kbs = sph(-kd);
-        } else if (n >= 3) {
-          Lmap<double> L = update_E_map(p.sca64, -s.kprev, sph(-kd));
-          s.Q = mul(s.Q, L);
-          lmap_normalise(s.Q, s.lq);
-        }
-        s.kprev = kd;
-      }
-      // scatter + free path with rejection (cpu.jl:194-235)
-      {
-        Fr<R> ks = sph(s.k);
-        for (int it = 0;; ++it) {
-          R ct = phase_sample(sca, rng.template u<R>());
-          R P[4];
-          phase_eval(sca, ct, P);
-          R eq = -P[1] * s.I[1] / (P[0] * s.I[0]);
-          R eu = -P[1] * s.I[2] / (P[0] * s.I[0]);
-          R ee = M<R>::sqrt(eq * eq + eu * eu);
-          R phi;
-          if (ee > R(1e-12)) {
-            R gam = M<R>::acos(clamp1(eq / ee));
-            if (eu < R(0)) gam = M<R>::two_pi - gam;
-            R eec = ee < R(1) ? ee : R(1);
-            R ma = M<R>::four_pi * rng.template u<R>() + gam - eec * M<R>::sin(gam);
-            R ea = kepler(ma, eec);
-            phi = R(0.5) * (ea - gam);
+          R* cr = cbrec + kCbRec * e_me;
+          const int cb_on = p.compute_cb && (near_up || near_lo);
+          Fr<double> kas = sph(kd);
+          V3<double> xa = {(double)s.x.x, (double)s.x.y, (double)s.x.z};
+          V3<double> kf0 = {p.kf0[0], p.kf0[1], p.kf0[2]};
+          if (cb_on && n > 1) {
+            // collect_cb n > 1 set-up (cpu.jl:53-81) from the O(1) state
+            double ta = dist64(p, xa, kf0, s.idx);
+            double tb = dist64(p, s.x1d, kf0, s.idx1);
+            double Id[4] = {(double)s.I[0], (double)s.I[1], (double)s.I[2], (double)s.I[3]};
+            double Iaf[4];
+            update_I(p.sca64, Id, kf0, kas, Iaf);
+            // reversed path: Q · L(-k_n | frame₀) · E_i, scaled by exp(lq + enorma)
+            Fr<double> f0 = {p.f0[0], p.f0[1], p.f0[2], p.f0[3]};
+            Lmap<double> L0 = update_E_map(p.sca64, -kd, f0);
+            Cx<double> Ei1 = {p.Ei[0], p.Ei[1]}, Ei2 = {p.Ei[2], p.Ei[3]};
+            Cx<double> u1 = L0.a * Ei1 + L0.b * Ei2, u2 = L0.c * Ei1 + L0.d * Ei2;
+            Cx<double> v1 = s.Q.a * u1 + s.Q.b * u2, v2 = s.Q.c * u1 + s.Q.d * u2;
+            double sc = exp(s.lq + s.enorma);
+            Cx<double> Eb1 = v1 * sc, Eb2 = v2 * sc;
+            Cx<double> Eaf1 = s.Ea1, Eaf2 = s.Ea2;
+            update_E(p.sca64, Eaf1, Eaf2, kf0, kas);
+            double raa = sqrt(Iaf[0] / (Eaf1.re * Eaf1.re + Eaf1.im * Eaf1.im + Eaf2.re * Eaf2.re +
+                                        Eaf2.im * Eaf2.im));
+            cr[0] = (R)(s.Ea1.re * raa); cr[1] = (R)(s.Ea1.im * raa);
+            cr[2] = (R)(s.Ea2.re * raa); cr[3] = (R)(s.Ea2.im * raa);
+            cr[4] = (R)(Eb1.re * raa); cr[5] = (R)(Eb1.im * raa);
+            cr[6] = (R)(Eb2.re * raa); cr[7] = (R)(Eb2.im * raa);
+            R ma_[8], mb_[8];
+            frame_matrix(Fr<R>{(R)kas.ct, (R)kas.st, (R)kas.cp, (R)kas.sp}, ma_);
+            frame_matrix(Fr<R>{(R)s.kbs.ct, (R)s.kbs.st, (R)s.kbs.cp, (R)s.kbs.sp}, mb_);
+            for (int c = 0; c < 8; ++c) { cr[8 + c] = ma_[c]; cr[16 + c] = mb_[c]; }
+            cr[24] = s.x.x; cr[25] = s.x.y; cr[26] = s.x.z;
+            cr[27] = (R)s.x1d.x; cr[28] = (R)s.x1d.y; cr[29] = (R)s.x1d.z;
+            cr[30] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
+            cr[31] = p.isinf[s.idx1] ? R(INFINITY) : p.tau[s.idx1];
+            double* dx = cbdx + kCbDx * e_me;
+            dx[0] = xa.x - s.x1d.x; dx[1] = xa.y - s.x1d.y; dx[2] = xa.z - s.x1d.z;
+            dx[3] = ta - tb;  // τab
+            cr[39] = R(2);
+          } else if (cb_on) {
+            // collect_cb n = 1 (cpu.jl:131-158)
+            R ma_[8];
+            frame_matrix(Fr<R>{(R)kas.ct, (R)kas.st, (R)kas.cp, (R)kas.sp}, ma_);
+            for (int c = 0; c < 8; ++c) cr[8 + c] = ma_[c];
+            cr[0] = s.I[0]; cr[1] = s.I[1]; cr[2] = s.I[2]; cr[3] = s.I[3];
+            cr[24] = s.x.x; cr[25] = s.x.y; cr[26] = s.x.z;
+            cr[30] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
+            cr[39] = R(1);
           } else {
-            phi = M<R>::two_pi * rng.template u<R>();
+            cr[39] = R(0);
           }
-          R st = sqrt0(R(1) - ct * ct);
-          R sph_, cph_;
-          M<R>::sincos(phi, &sph_, &cph_);
-          s.kp = ray_to_norm(V3<R>{st * cph_, st * sph_, ct}, ks);
-          R Ipt[4];
-          update_I(sca, s.I, s.kp, ks, Ipt);
-          R rr = s.I[0] / Ipt[0];
-          for (int c = 0; c < 4; ++c) s.Ip[c] = Ipt[c] * rr;
-          R t = -M<R>::log(rng.template u<R>());
-          s.xp = {s.x.x + t * s.kp.x, s.x.y + t * s.kp.y, s.x.z + t * s.kp.z};
-          int pb;
-          if (GEO == 0)
-            pb = s.xp.z > R(0) ? -1 : ((!p.isinf[s.idx] && s.xp.z < -p.tau[s.idx]) ? 1 : 0);
-          else
-            pb = M<R>::sqrt(dot(s.xp, s.xp)) > p.tau[0] ? -2 : 0;
-          s.idxp = s.idx + pb;
-          bool ok;
-          if (pb == -2) ok = false;
-          else if (pb == -1) {
-            ok = s.idxp >= 0;
-            if (ok) s.xp = {s.x.x, s.x.y, -p.tau[s.idx]};
-          } else if (pb == 1) {
-            ok = s.idxp <= p.nl - 1;
-            if (ok) s.xp = {s.x.x, s.x.y, R(0)};
-          } else ok = true;
-          if (ok) break;
-          ++cnt[kCntRej];
-          if (it >= (1 << 24)) { s.idxp = s.idx; s.xp = s.x; break; }
         }
       }
-      s.draw = rng.draw;
-      // absorption (cpu.jl:238-239)
-      R om = p.omega[s.idx];
-      Aacc[1] += (double)((R(1) - om) * s.I[0]);
-      for (int c = 0; c < 4; ++c) s.I[c] = s.I[c] * om;
-      // near flags (cpu.jl:245-246)
-      R dfirst, dlast;
-      if (GEO == 0) {
-        dfirst = s.idx == 0 ? M<R>::fabs(s.x.z) : R(INFINITY);
-        dlast = (s.idx == p.nl - 1 && !p.isinf[s.idx]) ? p.tau[s.idx] - M<R>::fabs(s.x.z) : R(INFINITY);
-      } else {
-        dfirst = p.tau[0] - M<R>::sqrt(dot(s.x, s.x));
-        dlast = R(INFINITY);
-      }
-      const int near_up = dfirst <= p.tau0;
-      const int near_lo = dlast <= p.tau0;
-      // RT event record
-      R* rec = rtrec + kRtRec * e_me;
-      frame_matrix(sph(s.k), rec);
-      rec[8] = s.I[0]; rec[9] = s.I[1]; rec[10] = s.I[2]; rec[11] = s.I[3];
-      if (GEO == 0) {
-        rec[12] = s.x.z;
-        rec[13] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
-        rec[14] = R(0);
-      } else {
-        rec[12] = s.x.x; rec[13] = s.x.y; rec[14] = s.x.z;
-      }
-      rec[15] = R(near_up | (near_lo << 1));
-      for (int w = 0; w < kWaves; ++w) { esc[(e_me * kWaves + w) * 2] = R(0); esc[(e_me * kWaves + w) * 2 + 1] = R(0); }
-      if (CB) {
-        R* cr = cbrec + kCbRec * e_me;
-        const int cb_on = p.compute_cb && (near_up || near_lo);
-        Fr<double> kas = sph(kd);
-        V3<double> xa = {(double)s.x.x, (double)s.x.y, (double)s.x.z};
-        V3<double> kf0 = {p.kf0[0], p.kf0[1], p.kf0[2]};
-        if (cb_on && n > 1) {
-          // collect_cb n > 1 set-up (cpu.jl:53-81) from the O(1) state
-          double ta = dist64(p, xa, kf0, s.idx);
-          double tb = dist64(p, s.x1d, kf0, s.idx1);
-          double Id[4] = {(double)s.I[0], (double)s.I[1], (double)s.I[2], (double)s.I[3]};
-          double Iaf[4];
-          update_I(p.sca64, Id, kf0, kas, Iaf);
-          // reversed path: Q · L(-k_n | frame₀) · E_i, scaled by exp(lq + enorma)
-          Fr<double> f0 = {p.f0[0], p.f0[1], p.f0[2], p.f0[3]};
-          Lmap<double> L0 = update_E_map(p.sca64, -kd, f0);
-          Cx<double> Ei1 = {p.Ei[0], p.Ei[1]}, Ei2 = {p.Ei[2], p.Ei[3]};
-          Cx<double> u1 = L0.a * Ei1 + L0.b * Ei2, u2 = L0.c * Ei1 + L0.d * Ei2;
-          Cx<double> v1 = s.Q.a * u1 + s.Q.b * u2, v2 = s.Q.c * u1 + s.Q.d * u2;
-          double sc = exp(s.lq + s.enorma);
-          Cx<double> Eb1 = v1 * sc, Eb2 = v2 * sc;
-          Cx<double> Eaf1 = s.Ea1, Eaf2 = s.Ea2;
-          update_E(p.sca64, Eaf1, Eaf2, kf0, kas);
-          double raa = sqrt(Iaf[0] / (Eaf1.re * Eaf1.re + Eaf1.im * Eaf1.im + Eaf2.re * Eaf2.re +
-                                      Eaf2.im * Eaf2.im));
-          cr[0] = (R)(s.Ea1.re * raa); cr[1] = (R)(s.Ea1.im * raa);
-          cr[2] = (R)(s.Ea2.re * raa); cr[3] = (R)(s.Ea2.im * raa);
-          cr[4] = (R)(Eb1.re * raa); cr[5] = (R)(Eb1.im * raa);
-          cr[6] = (R)(Eb2.re * raa); cr[7] = (R)(Eb2.im * raa);
-          R ma_[8], mb_[8];
-          Fr<R> kasR = {(R)kas.ct, (R)kas.st, (R)kas.cp, (R)kas.sp};
-          Fr<R> kbsR = {(R)s.kbs.ct, (R)s.kbs.st, (R)s.kbs.cp, (R)s.kbs.sp};
-          frame_matrix(kasR, ma_);
-          frame_matrix(kbsR, mb_);
-          for (int c = 0; c < 8; ++c) { cr[8 + c] = ma_[c]; cr[16 + c] = mb_[c]; }
-          cr[24] = s.x.x; cr[25] = s.x.y; cr[26] = s.x.z;
-          cr[27] = (R)s.x1d.x; cr[28] = (R)s.x1d.y; cr[29] = (R)s.x1d.z;
-          cr[30] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
-          cr[31] = p.isinf[s.idx1] ? R(INFINITY) : p.tau[s.idx1];
-          double* dx = cbdx + kCbDx * e_me;
-          dx[0] = xa.x - s.x1d.x; dx[1] = xa.y - s.x1d.y; dx[2] = xa.z - s.x1d.z;
-          dx[3] = ta - tb;  // τab
-          cr[39] = R(2);
-        } else if (cb_on) {
-          // collect_cb n = 1 (cpu.jl:131-158)
-          R ma_[8];
-          frame_matrix(Fr<R>{(R)kas.ct, (R)kas.st, (R)kas.cp, (R)kas.sp}, ma_);
-          for (int c = 0; c < 8; ++c) cr[8 + c] = ma_[c];
-          cr[0] = s.I[0]; cr[1] = s.I[1]; cr[2] = s.I[2]; cr[3] = s.I[3];
-          cr[24] = s.x.x; cr[25] = s.x.y; cr[26] = s.x.z;
-          cr[30] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
-          cr[39] = R(1);
-        } else {
-          cr[39] = R(0);
-        }
-      }
-    }
-    live = s.live;
-    if (live) *myslot = s;
-    if (tid == 0) misc[1] = E;
+      live = s.live;
+      if (owner && live) *myslot = s;
+      if (tid == 0) wcnt[31] = E;
     }
     __syncthreads();
-    const int E = misc[1];
+    const int E = wcnt[31];
 
     // ------------------------------- peel-off: bins × events (cpu.jl:6-31)
     for (int rg = 0; rg < p.nranges; ++rg) {
       const int jb = p.range_b[rg], je = p.range_e[rg];
       const int nrow = je - jb;
-      const int nb = nrow * p.Np;           // bins of this range
-      const int per = (nb + kBlock - 1) / kBlock;
-      const int flag = 1 << (p.range_kind[rg] == 1 ? 0 : 1);
+      const int nb = nrow * p.Np;  // bins of this range
+      const int per = (nb + kThreads - 1) / kThreads;
+      const int flag = p.range_kind[rg] == 1 ? 1 : 2;
       const int slot2 = p.range_kind[rg] == 1 ? 0 : 1;
       // full register chunks, then the remainder in passes of 4, 2 and 1
       int c0 = 0;
-      for (; c0 + kChunkMax <= per; c0 += kChunkMax)
-        rt_pass<kChunkMax, R, GEO>(p, sca, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
-                                   c0, flag, slot2);
-      if (kChunkMax > 4 && c0 + 4 <= per) {
-        rt_pass<4, R, GEO>(p, sca, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb, c0, flag, slot2);
+      for (; c0 + CH <= per; c0 += CH)
+        rt_pass<CH, R, GEO, MIE>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow,
+                                 nb, c0, flag, slot2);
+      if (CH > 4 && c0 + 4 <= per) {
+        rt_pass<4, R, GEO, MIE>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
+                                c0, flag, slot2);
         c0 += 4;
       }
       if (c0 + 2 <= per) {
-        rt_pass<2, R, GEO>(p, sca, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb, c0, flag, slot2);
+        rt_pass<2, R, GEO, MIE>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
+                                c0, flag, slot2);
         c0 += 2;
       }
       if (c0 < per)
-        rt_pass<1, R, GEO>(p, sca, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb, c0, flag, slot2);
+        rt_pass<1, R, GEO, MIE>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
+                                c0, flag, slot2);
     }
 
     // ------------------------------------------ CB bins × events (cpu.jl:38-160)
-    if (CB && p.compute_cb) {
-      const int nb = p.Ntb * p.Npb;
-      for (int lb = tid; lb < nb; lb += kBlock) {
-        const R* bk = p.cb_bin + 8 * lb;
-        const R kfx = bk[0], kfy = bk[1], kfz = bk[2], eh1 = bk[3], eh2 = bk[4];
-        const R c2p = bk[5], s2p = bk[6], ikz = bk[7];
-        const double* kk = p.cb_kik + 4 * lb;
-        const double kikx = kk[0], kiky = kk[1], kikz = kk[2];
-        R acc[4] = {R(0), R(0), R(0), R(0)};
-        for (int e = 0; e < E; ++e) {
-          const R* cr = cbrec + kCbRec * e;
-          const int kind = (int)cr[39];
-          if (kind == 0) continue;
-          const V3<R> kf = {kfx, kfy, kfz};
-          R o[4];
-          bool hit;
-          if (kind == 1) {
-            R m[8], I[4];
-            for (int c = 0; c < 8; ++c) m[c] = cr[8 + c];
-            for (int c = 0; c < 4; ++c) I[c] = cr[c];
-            V3<R> xa = {cr[24], cr[25], cr[26]};
-            R ta;
-            if (GEO == 0) ta = kfz > R(0) ? -xa.z * ikz : (xa.z + cr[30]) * -ikz;
-            else ta = geo.dist(xa, kf, 0);
-            hit = ta <= p.tau0;
-            if (hit) {
-              peel_I(sca, m, I, kfx, kfy, kfz, eh1, eh2, o);
-              R na = M<R>::exp(-ta);
-              for (int c = 0; c < 4; ++c) o[c] = o[c] * na;
-            }
-          } else {
-            V3<R> xa = {cr[24], cr[25], cr[26]};
-            V3<R> xb = {cr[27], cr[28], cr[29]};
-            R ta;
-            if (GEO == 0) ta = kfz > R(0) ? -xa.z * ikz : (xa.z + cr[30]) * -ikz;
-            else ta = geo.dist(xa, kf, 0);
-            hit = ta <= p.tau0;
-            if (hit) {
-              Cx<R> a1 = {cr[0], cr[1]}, a2 = {cr[2], cr[3]};
-              Cx<R> b1 = {cr[4], cr[5]}, b2 = {cr[6], cr[7]};
-              peel_E(sca, cr + 8, a1, a2, kfx, kfy, kfz, eh1, eh2);
-              peel_E(sca, cr + 16, b1, b2, kfx, kfy, kfz, eh1, eh2);
-              const double* dx = cbdx + kCbDx * e;
-              double dphi = p.kl1 * (kikx * dx[0] + kiky * dx[1] + kikz * dx[2]);
-              R tab = (R)dx[3];
-              R na = M<R>::exp(R(-0.5) * ta);
-              a1 = a1 * na; a2 = a2 * na;
-              R Ia[4];
-              I_from_E(a1, a2, Ia);
-              R tb;
-              if (GEO == 0) tb = kfz > R(0) ? -xb.z * ikz : (xb.z + cr[31]) * -ikz;
-              else tb = geo.dist(xb, kf, 0);
-              if (M<R>::fabs(tb + tab) <= p.tau0) {
-                R nb_ = M<R>::exp(R(-0.5) * (tb + tab));
-                b1 = b1 * nb_; b2 = b2 * nb_;
-              } else {
-                b1 = {R(0), R(0)}; b2 = {R(0), R(0)};
-              }
-              R Ib[4];
-              I_from_E(b1, b2, Ib);
-              R cph, sph_;
-              phase_cs(dphi, cph, sph_);
-              Cx<R> ep = {cph, sph_};
-              Cx<R> e1 = a1 * ep + b1, e2 = a2 * ep + b2;
-              I_from_E(e1, e2, o);
-              R rr = Ia[0] / (Ia[0] + Ib[0]);
-              for (int c = 0; c < 4; ++c) o[c] = o[c] * rr;
-            }
-          }
-          if (hit) {
-            rotate(o, c2p, s2p);
-            for (int c = 0; c < 4; ++c) acc[c] += o[c];
-          }
-        }
-        const int64_t ob = 5 + p.nrt + 4 * (int64_t)lb;
-        for (int c = 0; c < 4; ++c) kahan(slab + ob + c, slabc + ob + c, (double)acc[c]);
-      }
-    }
+    if (CB && p.compute_cb) cb_pass<R, GEO, MIE>(p, ldsF, Sbins, cbrec, cbdx, slab, slabc, E, tid);
     __syncthreads();
 
     // ---------------------- weight update and termination (cpu.jl:263-272)
-    if (live) {
+    if (owner && live) {
       Slot<R> s = *myslot;
       R Iref = R(0), Itrans = R(0);
       for (int w = 0; w < kWaves; ++w) {
         Iref += esc[(e_me * kWaves + w) * 2];
         Itrans += esc[(e_me * kWaves + w) * 2 + 1];
       }
-      const int fl = (int)rtrec[kRtRec * e_me + 15];
+      const int fl = (int)rtrec[kRtRec * e_me + 16];
       for (int rg = 0; rg < p.nranges; ++rg)
         if (fl & (p.range_kind[rg] == 1 ? 1 : 2))
           cnt[kCntRt] += (unsigned long long)(p.range_e[rg] - p.range_b[rg]) * p.Np;
@@ -782,6 +845,7 @@ __global__ __launch_bounds__(kBlock) void coop_kernel(const CoopParams<R>* __res
       Aacc[3] += (double)Itrans;
       R Iesc = Iref + Itrans;
       R sc = s.I[0] - Iesc;
+#pragma unroll
       for (int c = 0; c < 4; ++c) s.I[c] = s.Ip[c] * sc / s.Ip[0];
       s.k = s.kp; s.x = s.xp; s.idx = s.idxp;
       ++cnt[kCntEvents];
@@ -797,12 +861,13 @@ __global__ __launch_bounds__(kBlock) void coop_kernel(const CoopParams<R>* __res
 
   // ---------------------------------------- energy budget and counters
   {
-    double* buf = (double*)rtrec;  // reuse the record area: 5·kBlock doubles
-    for (int c = 0; c < 5; ++c) buf[c * kBlock + tid] = Aacc[c];
+    double* buf = (double*)rtrec;  // reuse the record area: 5·SLOTS doubles
+    if (owner)
+      for (int c = 0; c < 5; ++c) buf[c * SLOTS + tid] = Aacc[c];
     __syncthreads();
     if (tid < 5) {
       double sum = 0.0, cmp = 0.0;
-      for (int i = 0; i < kBlock; ++i) kahan(&sum, &cmp, buf[tid * kBlock + i]);
+      for (int i = 0; i < SLOTS; ++i) kahan(&sum, &cmp, buf[tid * SLOTS + i]);
       slab[tid] = sum;
       slabc[tid] = cmp;
     }
@@ -882,6 +947,7 @@ void h_E_from_I(const double I[4], double E[4]) {
   E[0] = A1; E[1] = A1 * 0.0; E[2] = A2 * std::cos(ph); E[3] = A2 * std::sin(ph);
 }
 
+// Upload a scatterer's tables in T, F and S padded with one zero node.
 template <typename R>
 void fill_sca(ScaView<R>& v, const mcrt_scatterer* s, const DevAlloc& alloc, int* rc, std::string* err) {
   v.kind = s->kind == MCRT_SCATTERER_MIE ? 1 : 0;
@@ -891,7 +957,8 @@ void fill_sca(ScaView<R>& v, const mcrt_scatterer* s, const DevAlloc& alloc, int
   v.csrn = v.F = v.S = nullptr;
   if (!v.kind) return;
   const int N = s->n_theta;
-  std::vector<R> F(4 * (size_t)N), S(4 * (size_t)N), C((size_t)s->random_points + 1);
+  std::vector<R> F(4 * (size_t)(N + 1), R(0)), S(4 * (size_t)(N + 1), R(0)),
+      C((size_t)s->random_points + 1);
   for (int i = 0; i < N; ++i) {
     F[4 * i] = (R)s->F11[i]; F[4 * i + 1] = (R)s->F12[i];
     F[4 * i + 2] = (R)s->F33[i]; F[4 * i + 3] = (R)s->F34[i];
@@ -913,10 +980,22 @@ void fill_sca(ScaView<R>& v, const mcrt_scatterer* s, const DevAlloc& alloc, int
   if (v.S) v.csrn = up(C);
 }
 
+// LDS bytes of one workgroup (the layout of coop_kernel).
+template <typename R> size_t lds_plan(bool mie, bool cb, int nt, int rp) {
+  constexpr int SLOTS = Shape<R>::kSlots;
+  auto al = [](size_t b) { return (b + 15) & ~size_t(15); };
+  size_t b = al(sizeof(R) * kRtRec * SLOTS);
+  if (cb) b += al(sizeof(R) * kCbRec * SLOTS) + al(sizeof(double) * kCbDx * SLOTS);
+  b += al(sizeof(R) * SLOTS * kWaves * 2) + al(sizeof(int) * 32);
+  if (mie) b += al(sizeof(R) * 4 * (nt + 1)) + al(sizeof(R) * (rp + 1));
+  if (mie && cb && Shape<R>::kStageS) b += al(sizeof(R) * 4 * (nt + 1));
+  return b;
+}
+
 template <typename R>
-int create_impl(CoopPlan& cp, int cu_count, const mcrt_config* cfg, const mcrt_geometry* g,
-                const double I0[4], uint64_t seed, const DevAlloc& alloc, unsigned long long* dcnt,
-                std::string* err) {
+int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* cfg,
+                const mcrt_geometry* g, const double I0[4], uint64_t seed, const DevAlloc& alloc,
+                unsigned long long* dcnt, std::string* err) {
   CoopParams<R> P;
   std::memset(&P, 0, sizeof P);
   // one scatterer for all layers
@@ -927,14 +1006,21 @@ int create_impl(CoopPlan& cp, int cu_count, const mcrt_config* cfg, const mcrt_g
         *err = "the cooperative forms need one scatterer shared by all layers (use fp64_plain)";
         return MCRT_ERR_UNSUPPORTED;
       }
+  const bool mie = s0->kind == MCRT_SCATTERER_MIE;
+  const bool cb = cfg->compute_cb != 0;
+  const size_t lds = lds_plan<R>(mie, cb, mie ? s0->n_theta : 0, mie ? s0->random_points : 0);
+  if (lds > lds_max) {
+    *err = "the Mie tables do not fit in LDS beside the event records (reduce Nθ or random_points, "
+           "or use fp64_plain)";
+    return MCRT_ERR_UNSUPPORTED;
+  }
   int layer_slot[kMaxLayers] = {};
   GeoDev gd = make_geo(g, layer_slot);
   P.N = cfg->number_of_rays;
   P.maxscat = cfg->maximum_scattering_times;
   P.min_int = (R)cfg->minimum_intensity;
   P.tau0 = (R)cfg->tau0;
-  P.tau0d = cfg->tau0;
-  P.compute_cb = cfg->compute_cb ? 1 : 0;
+  P.compute_cb = cb ? 1 : 0;
   P.geo_kind = gd.kind;
   P.nl = gd.nl;
   for (int i = 0; i < gd.nl; ++i) {
@@ -947,9 +1033,9 @@ int create_impl(CoopPlan& cp, int cu_count, const mcrt_config* cfg, const mcrt_g
   if (rc) return rc;
   fill_sca<double>(P.sca64, s0, alloc, &rc, err);
   if (rc) return rc;
-  if (P.sca.kind) P.sca.scale = (R)((s0->n_theta - 1) / 3.141592653589793);
+  P.scale = mie ? (R)((s0->n_theta - 1) / 3.141592653589793) : R(0);
   const int Nt = (int)cfg->n_theta, Np = (int)cfg->n_phi;
-  const int Ntb = cfg->compute_cb ? (int)cfg->n_theta_b : 0, Npb = cfg->compute_cb ? (int)cfg->n_phi_b : 0;
+  const int Ntb = cb ? (int)cfg->n_theta_b : 0, Npb = cb ? (int)cfg->n_phi_b : 0;
   P.Nt = Nt; P.Np = Np; P.Ntb = Ntb; P.Npb = Npb;
   if (g->kind == MCRT_GEOMETRY_PLANE) {
     // lower_theta_range 1:Nθ÷2 (transmission), upper Nθ÷2+1:Nθ (reflection)
@@ -962,7 +1048,7 @@ int create_impl(CoopPlan& cp, int cu_count, const mcrt_config* cfg, const mcrt_g
   // incident frame and CB constants (FP64, host)
   double k0[3], f0[4];
   incident(g, k0, f0);
-  for (int i = 0; i < 3; ++i) { P.k0[i] = (R)k0[i]; P.k0d[i] = k0[i]; P.kf0[i] = -k0[i]; }
+  for (int i = 0; i < 3; ++i) { P.k0[i] = (R)k0[i]; P.kf0[i] = -k0[i]; }
   for (int i = 0; i < 4; ++i) { P.f0[i] = f0[i]; P.I0[i] = (R)I0[i]; }
   h_E_from_I(I0, P.Ei);
   {
@@ -970,7 +1056,6 @@ int create_impl(CoopPlan& cp, int cu_count, const mcrt_config* cfg, const mcrt_g
     P.norm0 = std::sqrt(a * a + b * b);
   }
   HFr kfs = h_sph(P.kf0);
-  P.kfs[0] = kfs.ct; P.kfs[1] = kfs.st; P.kfs[2] = kfs.cp; P.kfs[3] = kfs.sp;
   // RT bin constants: direction (cpu.jl:13), basis (get_basis), inorm, 1/k_z
   std::vector<R> rtb(8 * (size_t)Nt * Np);
   for (int j2 = 0; j2 < Np; ++j2)
@@ -1027,33 +1112,17 @@ int create_impl(CoopPlan& cp, int cu_count, const mcrt_config* cfg, const mcrt_g
   if (rc) return rc;
   P.seed = seed;
   P.instance = 0;
-  P.chunk = sizeof(R) == 4 ? 8 : 4;  // informational: the kernel's register chunk
   P.nrt = 4 * (int64_t)Nt * Np;
   P.ncb = 4 * (int64_t)cfg->n_theta_b * cfg->n_phi_b;
   P.nout = 5 + P.nrt + P.ncb;
   P.counters = dcnt;
-  // LDS plan
-  const bool mie = P.sca.kind == 1;
-  const bool cb = cfg->compute_cb != 0;
-  auto al = [](size_t b) { return (b + 15) & ~size_t(15); };
-  size_t fixed = al(sizeof(R) * kRtRec * kBlock) + al(sizeof(R) * kBlock * kWaves * 2) +
-                 al(sizeof(int) * 16) + al(sizeof(int) * 8);
-  if (cb) fixed += al(sizeof(R) * kCbRec * kBlock) + al(sizeof(double) * kCbDx * kBlock);
-  size_t tabF = mie ? al(sizeof(R) * 4 * s0->n_theta) + al(sizeof(R) * (s0->random_points + 1)) : 0;
-  size_t tabS = (mie && cb) ? al(sizeof(R) * 4 * s0->n_theta) : 0;
-  const size_t kLds = 160 * 1024;
-  P.stage_F = 0;
-  P.stage_S = 0;
-  size_t lds = fixed;
-  if (mie && lds + tabF <= kLds) { P.stage_F = 1; lds += tabF; }
-  if (mie && cb && P.stage_F && lds + tabS <= kLds) { P.stage_S = 1; lds += tabS; }
+  // launch shape: LDS bounds the workgroups per CU; at most one per SLOTS rays
   cp.lds_bytes = lds;
-  const int wg_per_cu = std::max(1, (int)(kLds / std::max<size_t>(lds, 1)));
-  int64_t want = (int64_t)cu_count * std::min(wg_per_cu, 2);
-  int64_t byrays = (P.N + kBlock - 1) / kBlock;
+  const int wg_per_cu = std::max<int>(1, (int)(lds_max / std::max<size_t>(lds, 1)));
+  const int64_t want = (int64_t)cu_count * std::min(wg_per_cu, 2);
+  const int64_t byrays = (P.N + Shape<R>::kSlots - 1) / Shape<R>::kSlots;
   cp.blocks = (int)std::max<int64_t>(1, std::min(want, byrays));
-  cp.threads = kBlock;
-  // slabs
+  cp.threads = kThreads;
   hipError_t e;
   cp.nslab = cp.blocks;
   cp.slab_stride = 2 * P.nout;
@@ -1061,7 +1130,7 @@ int create_impl(CoopPlan& cp, int cu_count, const mcrt_config* cfg, const mcrt_g
   cp.slabs = (double*)alloc(slab_bytes, &e);
   if (!cp.slabs) { *err = "coop: slab allocation failed"; return MCRT_ERR_NOMEM; }
   P.slabs = cp.slabs;
-  P.slot_mem = alloc(sizeof(Slot<R>) * (size_t)cp.blocks * kBlock, &e);
+  P.slot_mem = alloc(sizeof(Slot<R>) * (size_t)cp.blocks * Shape<R>::kSlots, &e);
   if (!P.slot_mem) { *err = "coop: slot-state allocation failed"; return MCRT_ERR_NOMEM; }
   cp.nout = P.nout;
   cp.nrt = P.nrt;
@@ -1097,15 +1166,16 @@ hipError_t launch_r(const CoopPlan& cp, hipStream_t s) {
 
 }  // namespace
 
-int coop_create(CoopPlan& cp, hipStream_t, int cu_count, const mcrt_config* cfg,
+int coop_create(CoopPlan& cp, hipStream_t, int cu_count, size_t lds_max, const mcrt_config* cfg,
                 const mcrt_geometry* geom, const double I0[4], uint64_t seed, bool mixed,
                 const DevAlloc& alloc, double* dA, double* dIrt, double* dIcb,
                 unsigned long long* dcnt, std::string* err) {
   cp.mixed = mixed;
   cp.dA = dA; cp.dIrt = dIrt; cp.dIcb = dIcb;
   if (cu_count <= 0) cu_count = 256;
-  return mixed ? create_impl<float>(cp, cu_count, cfg, geom, I0, seed, alloc, dcnt, err)
-               : create_impl<double>(cp, cu_count, cfg, geom, I0, seed, alloc, dcnt, err);
+  if (lds_max == 0) lds_max = 160 * 1024;
+  return mixed ? create_impl<float>(cp, cu_count, lds_max, cfg, geom, I0, seed, alloc, dcnt, err)
+               : create_impl<double>(cp, cu_count, lds_max, cfg, geom, I0, seed, alloc, dcnt, err);
 }
 
 hipError_t coop_prepare(CoopPlan& cp, hipStream_t s) {

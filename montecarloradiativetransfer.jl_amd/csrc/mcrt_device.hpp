@@ -40,7 +40,7 @@ template <> struct M<double> {
   static constexpr double kepler_tol = 1e-12;
 };
 template <> struct M<float> {
-  static MCRT_DEV float sqrt(float x) { return __builtin_sqrtf(x); }
+  static MCRT_DEV float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }  // v_sqrt_f32
   static MCRT_DEV float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
   // 2^(x·log2 e) on the hardware exp2 (v_exp_f32).
   static MCRT_DEV float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
