@@ -51,6 +51,26 @@ def run_form(cfg, g, seed, I0, form, steps, warmup):
     return dict(wall_s=t1 - t0, kernel_ms=kms, stats=st, out=out)
 
 
+def copy_calibration(mib: int) -> int:
+    """hipMemcpy device→device of mib MiB: one copy kernel that reads and writes
+    exactly mib MiB, for scaling the FETCH_SIZE/WRITE_SIZE counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_mib": mib}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,7 +88,12 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=8192,
                     help="rays of the oracle CPU baseline run")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (the known-size kernel "
+                         "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(json.dumps({"error": "this build targets one GPU; --gpus must be 1 (rays shard "
                           "across GPUs by disjoint ray ranges with one final bin reduction, "
@@ -110,7 +135,7 @@ def main() -> int:
         f"(algorithmic), events/ray {st['events'] / N:.2f}")
 
     compare = {}
-    for form in [f for f in args.compare.split(",") if f and f != head]:
+    for form in [f for f in args.compare.split(",") if f and f not in (head, "none")]:
         n = args.plain_sample if form == "fp64_plain" else args.fp64_sample
         n = min(n, N)
         try:

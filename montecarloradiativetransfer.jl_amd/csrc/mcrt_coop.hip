@@ -99,10 +99,36 @@ template <typename R> struct CoopParams {
 
 namespace {
 
+// Sum over the 64 lanes of a wave with DPP row operations (no LDS traffic):
+// row_shr 1, 2, 4, 8 leave each row's total in its lane 15, row_bcast 15
+// and 31 carry the totals into lane 63, which is broadcast by readlane.
+template <int CTRL, int ROWS> MCRT_DEV int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS> MCRT_DEV float dpp_add(float v) {
+  return v + __int_as_float(dpp_i<CTRL, ROWS>(__float_as_int(v)));
+}
+template <int CTRL, int ROWS> MCRT_DEV double dpp_add(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i<CTRL, ROWS>((int)(b & 0xffffffffll));
+  const int hi = dpp_i<CTRL, ROWS>((int)(b >> 32));
+  return v + __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+MCRT_DEV float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+MCRT_DEV double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 template <typename R> MCRT_DEV R wave_sum(R v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v = dpp_add<0x111, 0xf>(v);  // row_shr:1
+  v = dpp_add<0x112, 0xf>(v);  // row_shr:2
+  v = dpp_add<0x114, 0xf>(v);  // row_shr:4
+  v = dpp_add<0x118, 0xf>(v);  // row_shr:8
+  v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return lane63(v);
 }
 
 // Kahan-compensated add of x into (s, c).
