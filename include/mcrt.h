@@ -212,6 +212,16 @@ int mcrt_plan_read(mcrt_plan* plan, double A[5], double* Irt, double* Icb,
 int mcrt_plan_kernel_ms(mcrt_plan* plan, double* ms);
 void mcrt_plan_destroy(mcrt_plan* plan);
 
+/* Monte Carlo standard error of every output of the last launch (same
+ * layouts and normalisation as mcrt_plan_read), by batch means over the
+ * workgroup ray ranges: *batches ray ranges of near-equal size, each summed
+ * separately on the device.  The reference reports no error estimate; this
+ * is the optional `sigma` output of simulate.  Cooperative forms only
+ * (MCRT_PRECISION_FP64, MCRT_PRECISION_MIXED; -2 for FP64_PLAIN).  NaN when
+ * the solve ran as a single batch.  Any output may be NULL. */
+int mcrt_plan_read_sigma(mcrt_plan* plan, double A_sigma[5], double* Irt_sigma,
+                         double* Icb_sigma, int32_t* batches);
+
 #ifdef __cplusplus
 }
 #endif

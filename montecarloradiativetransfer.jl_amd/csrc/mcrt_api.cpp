@@ -71,6 +71,7 @@ struct mcrt_plan {
   int plain_blocks = 0;
   // cooperative forms
   mcrt::CoopPlan coop{};
+  double* dsig = nullptr;  // batch-means standard errors (allocated on first use)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool launched = false;
   std::chrono::steady_clock::time_point t_launch;
@@ -500,6 +501,34 @@ extern "C" int mcrt_plan_read(mcrt_plan* P, double A[5], double* Irt, double* Ic
     st->kernel_ms = ms;
     st->total_ms = P->total_ms;
   }
+  return MCRT_OK;
+}
+
+extern "C" int mcrt_plan_read_sigma(mcrt_plan* P, double A_sigma[5], double* Irt_sigma,
+                                    double* Icb_sigma, int32_t* batches) {
+  if (!P) return fail(nullptr, MCRT_ERR_INVALID, "plan is NULL");
+  mcrt_ctx* ctx = P->ctx;
+  if (P->precision == MCRT_PRECISION_FP64_PLAIN)
+    return fail(ctx, MCRT_ERR_UNSUPPORTED,
+                "standard errors come from the workgroup slabs of the cooperative forms "
+                "(precision fp64 or mixed)");
+  int rc = mcrt_plan_wait(P);
+  if (rc) return rc;
+  const int64_t nout = P->coop.nout;
+  if (!P->dsig) {
+    hipError_t e;
+    P->dsig = static_cast<double*>(P->alloc(sizeof(double) * (size_t)nout, &e));
+    if (!P->dsig) return fail(ctx, MCRT_ERR_NOMEM, "sigma buffer: %s", hipGetErrorString(e));
+  }
+  HIP_TRY(ctx, mcrt::coop_sigma(P->coop, ctx->stream, P->N, P->dsig));
+  std::vector<double> h((size_t)nout);
+  HIP_TRY(ctx, hipMemcpyAsync(h.data(), P->dsig, sizeof(double) * (size_t)nout,
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (A_sigma) for (int i = 0; i < 5; ++i) A_sigma[i] = h[i];
+  if (Irt_sigma) for (int64_t i = 0; i < P->nrt; ++i) Irt_sigma[i] = h[5 + i];
+  if (Icb_sigma) for (int64_t i = 0; i < P->ncb; ++i) Icb_sigma[i] = h[5 + P->nrt + i];
+  if (batches) *batches = P->coop.nslab;
   return MCRT_OK;
 }
 

@@ -46,5 +46,8 @@ hipError_t coop_prepare(CoopPlan& cp, hipStream_t stream);
 hipError_t coop_launch(CoopPlan& cp, hipStream_t stream);
 // Enqueue the reduction of the slabs into A, Irt, Icb.
 hipError_t coop_reduce(CoopPlan& cp, hipStream_t stream);
+// Enqueue the batch-means standard error of the N-normalised outputs
+// (sig[nout] in the slab order: A[5], Irt, Icb).
+hipError_t coop_sigma(CoopPlan& cp, hipStream_t stream, int64_t N, double* sig);
 
 }  // namespace mcrt

@@ -43,6 +43,8 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kRtRec = 20;  // R words of an RT event record
 constexpr int kCbRec = 40;  // R words of a CB event record
 constexpr int kCbDx = 4;    // doubles of a CB event record (x_a - x_b, τab)
+constexpr int kBinW = 11;     // R words of a bin used by the evaluation
+constexpr int kBinStride = 12;  // R words per bin in the bin tables
 
 // Per-precision launch shape: ray slots per workgroup and RT bins per
 // register chunk.
@@ -63,6 +65,9 @@ template <typename R> struct CoopParams {
   int64_t maxscat;
   R min_int;
   R tau0;
+  R tau0k;      // tau0 · ExpK<R>::k
+  R acs[8];     // FP32 arccos coefficients times (Nθ_table - 1)/π
+  R pis;        // π · (Nθ_table - 1)/π
   int compute_cb;
   int geo_kind;  // 0 plane, 1 sphere
   int nl;
@@ -79,8 +84,8 @@ template <typename R> struct CoopParams {
   int Nt, Np, Ntb, Npb;
   int nranges;
   int range_b[2], range_e[2], range_kind[2];  // kind 0: lower (A4), 1: upper (A3)
-  const R* rt_bin;       // [Nt*Np][8]: kx ky kz Eh1 Eh2 inorm 1/kz 0, bin = j1 + Nt*j2
-  const R* cb_bin;       // [Ntb*Npb][8]: kfx kfy kfz Eh1 Eh2 cos2ψ sin2ψ 1/kfz
+  const R* rt_bin;       // [Nt*Np][12]: k, Eh1 Eh2, Eh × k, inorm, 1/kz; bin = j1 + Nt*j2
+  const R* cb_bin;       // [Ntb*Npb][12]: kf, Eh1 Eh2, Eh × kf, cos2ψ sin2ψ, 1/kfz
   const double* cb_kik;  // [Ntb*Npb][4]: k_i + k_f (FP64)
   R I0[4];
   R k0[3];
@@ -139,27 +144,45 @@ MCRT_DEV void kahan(double* s, double* c, double x) {
   *s = t;
 }
 
-// arccos.  FP32: Abramowitz & Stegun 4.4.46 (Hastings), |error| ≤ 2e-8 on
-// [0, 1], with acos(-x) = π - acos(x).  FP64: the library acos.
-MCRT_DEV double acos_t(double x) { return ::acos(x); }
-MCRT_DEV float acos_t(float x) {
-  float a = __builtin_fabsf(x);
-  float p = -0.0012624911f;
-  p = __builtin_fmaf(p, a, 0.0066700901f);
-  p = __builtin_fmaf(p, a, -0.0170881256f);
-  p = __builtin_fmaf(p, a, 0.0308918810f);
-  p = __builtin_fmaf(p, a, -0.0501743046f);
-  p = __builtin_fmaf(p, a, 0.0889789874f);
-  p = __builtin_fmaf(p, a, -0.2145988016f);
-  p = __builtin_fmaf(p, a, 1.5707963050f);
-  float r = __builtin_amdgcn_sqrtf(1.0f - a) * p;
-  return x < 0.0f ? 3.14159265358979f - r : r;
+// Table coordinate sn = acos(μ)·(Nθ-1)/π of a phase / amplitude lookup
+// (mie.jl:60-62).  FP32: Abramowitz & Stegun 4.4.46 (Hastings),
+// acos(a) = √(1-a)·p(a) on [0, 1] with |error| ≤ 2e-8, its coefficients
+// pre-multiplied by (Nθ-1)/π, and acos(-a) = π - acos(a).  FP64: the library acos.
+template <typename R> MCRT_DEV R theta_index(const CoopParams<R>& p, R mu);
+template <> MCRT_DEV double theta_index(const CoopParams<double>& p, double mu) {
+  return ::acos(clamp1(mu)) * p.scale;
+}
+template <> MCRT_DEV float theta_index(const CoopParams<float>& p, float mu) {
+  const float a = __builtin_fabsf(mu);
+  float c = p.acs[0];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) c = __builtin_fmaf(c, a, p.acs[k]);
+  const float r = __builtin_amdgcn_sqrtf(__builtin_fmaxf(1.0f - a, 0.0f)) * c;
+  return mu < 0.0f ? p.pis - r : r;
+}
+
+// √max(x, 0): one max and the hardware square root in FP32.
+MCRT_DEV float sqrt_pos(float x) { return __builtin_amdgcn_sqrtf(__builtin_fmaxf(x, 0.0f)); }
+MCRT_DEV double sqrt_pos(double x) { return ::sqrt(x < 0.0 ? 0.0 : x); }
+
+// Split sn into the node index i0 ≤ imax and the fraction f = sn - i0.
+MCRT_DEV int node_of(float sn, int imax, float& f) {
+  const float s = __builtin_fminf(sn, (float)imax);
+  f = __builtin_amdgcn_fractf(s);
+  return (int)s;
+}
+MCRT_DEV int node_of(double sn, int imax, double& f) {
+  const double s = sn < (double)imax ? sn : (double)imax;
+  const int i = (int)s;
+  f = s - (double)i;
+  return i;
 }
 
 // Phase matrix (P11, P12, P33, P34) at cosθ = mu: Rayleigh (rayleigh.jl:12-17)
-// or a lookup in the padded LDS table F (mie.jl:60-80, linear in θ).
+// or a lookup in the padded LDS table F, linear in θ between nodes i0 and
+// i0 + 1 (mie.jl:60-80).
 template <typename R, bool MIE>
-MCRT_DEV void phase_tab(const R* F, R scale, int imax, R mu, R P[4]) {
+MCRT_DEV void phase_tab(const CoopParams<R>& p, const R* F, R mu, R P[4]) {
   if constexpr (!MIE) {
     R m2 = mu * mu;
     P[0] = R(0.75) * (R(1) + m2);
@@ -167,33 +190,29 @@ MCRT_DEV void phase_tab(const R* F, R scale, int imax, R mu, R P[4]) {
     P[2] = R(1.5) * mu;
     P[3] = R(0);
   } else {
-    R sn = acos_t(clamp1(mu)) * scale;
-    int i0 = (int)sn;
-    i0 = i0 > imax ? imax : i0;
-    R w1 = sn - R(i0);
-    R w0 = R(i0 + 1) - sn;
+    R f;
+    const int i0 = node_of(theta_index(p, mu), p.sca.Nt - 1, f);
     const R* a = F + 4 * i0;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) P[c] = w0 * a[c] + w1 * a[4 + c];
+    for (int c = 0; c < 4; ++c) P[c] = a[c] + f * (a[4 + c] - a[c]);
   }
 }
 
 // Amplitude matrix diag(S11, S22) (rayleigh.jl:20-23; mie.jl:83-99 with its
-// weights (msn+1-sn, sn-msn): the zero pad node reproduces the skipped term).
+// weights (msn+1-sn, sn-msn) = (2-f, f-1) on nodes i0, i0 + 1: the zero pad
+// node reproduces the skipped term at θ = π).
 template <typename R, bool MIE>
-MCRT_DEV void amp_tab(const R* S, R scale, int imax, R mu, Cx<R>& S11, Cx<R>& S22) {
+MCRT_DEV void amp_tab(const CoopParams<R>& p, const R* S, R mu, Cx<R>& S11, Cx<R>& S22) {
   if constexpr (!MIE) {
     S11 = {R(0), R(1)};
     S22 = {mu * R(0), mu};
   } else {
-    R sn = acos_t(clamp1(mu)) * scale;
-    int i0 = (int)sn;
-    i0 = i0 > imax ? imax : i0;
-    R w0 = R(i0 + 2) - sn;
-    R w1 = sn - R(i0 + 1);
+    R f;
+    const int i0 = node_of(theta_index(p, mu), p.sca.Nt - 1, f);
+    const R g = R(1) - f;
     const R* a = S + 4 * i0;
-    S11 = {w0 * a[0] + w1 * a[4], w0 * a[1] + w1 * a[5]};
-    S22 = {w0 * a[2] + w1 * a[6], w0 * a[3] + w1 * a[7]};
+    S11 = {a[0] + g * (a[0] - a[4]), a[1] + g * (a[1] - a[5])};
+    S22 = {a[2] + g * (a[2] - a[6]), a[3] + g * (a[3] - a[7])};
   }
 }
 
@@ -223,79 +242,97 @@ template <typename R> MCRT_DEV double dist64(const CoopParams<R>& p, V3<double> 
 }
 
 // Frame rotation matrix of ray_to_norm (utils.jl:30-34): ray_to_norm(v) = M v,
-// norm_to_ray(v) = Mᵀ v.  Stored as 8 words (M21 = 0).
+// norm_to_ray(v) = Mᵀ v.  Stored as 8 words (M21 = 0); its columns are the
+// frame's θ̂, ϕ̂ and k.
 template <typename R> MCRT_DEV void frame_matrix(Fr<R> f, R m[8]) {
   m[0] = f.ct * f.cp; m[1] = -f.sp; m[2] = f.st * f.cp;
   m[3] = f.ct * f.sp; m[4] = f.cp;  m[5] = f.st * f.sp;
   m[6] = -f.st;       m[7] = f.ct;
 }
 
-// The frame part shared by update_I and update_E (utils.jl:92-105) for a
-// direction k with basis vector Eh (from get_basis(k)) and an incident frame
-// matrix m: returns μ = cosθₖ, cos/sin ϕₖ, cosψ and Eh·Evk.  Branch-free: the
-// polar case of spherical_from_cartesian (utils.jl:12-14) is a select.
+// Scattering plane of a bin direction kb (basis Eh, G = Eh × kb, both
+// precomputed per bin) seen from an event frame m, as needed by update_I
+// (utils.jl:91-112) and update_E (utils.jl:170-192).  With r = norm_to_ray(kb)
+// and ρ² = r₁² + r₂² = 1 - μ²:
+//   cosϕₖ = r₁/ρ, sinϕₖ = r₂/ρ, μ = r₃,
+//   cosψ = Eh·Ehₖ = k·(Eh × kb)/ρ = pn/ρ   (Binet-Cauchy, θ̂ × ϕ̂ = k),
+//   Eh·Evₖ = -(k·Eh)/ρ = -w/ρ             (Eh ⟂ kb, |kb| = 1),
+// so that every rotation of a peel-off is a ratio over the one factor
+// inv = 1/ρ² and needs no square root.  The polar branch of
+// spherical_from_cartesian (utils.jl:12-14, x² + y² < 1e-24: ϕₖ = 0,
+// θₖ ∈ {0, π}) is taken by a wave-uniform test, since exact alignment of a
+// bin with an event direction is rare (a divergent branch that is skipped
+// when no lane of the wave takes it).
+template <typename R> struct Plane { R mu, rx, ry, pn, w, inv; };
 template <typename R>
-MCRT_DEV void frame_part(const R* m, R kx, R ky, R kz, R eh1, R eh2, R& mu, R& cpk, R& spk,
-                         R& cpsi, R& ehev) {
-  R rx = m[0] * kx + m[3] * ky + m[6] * kz;
-  R ry = m[1] * kx + m[4] * ky;
-  R rz = m[2] * kx + m[5] * ky + m[7] * kz;
-  R s2 = rx * rx + ry * ry;
-  const bool polar = s2 < R(1e-24);
-  R inv = M<R>::rsqrt(polar ? R(1) : s2);
-  cpk = polar ? R(1) : rx * inv;
-  spk = polar ? R(0) : ry * inv;
-  R ctk = polar ? jl_sign(rz) : rz;
-  R stk = polar ? R(0) : sqrt0(R(1) - rz * rz);
-  mu = rz;
-  R ehk1 = m[0] * spk - m[1] * cpk;
-  R ehk2 = m[3] * spk - m[4] * cpk;
-  R a = ctk * cpk, b = ctk * spk;
-  R evk1 = m[0] * a + m[1] * b - m[2] * stk;
-  R evk2 = m[3] * a + m[4] * b - m[5] * stk;
-  cpsi = eh1 * ehk1 + eh2 * ehk2;
-  ehev = eh1 * evk1 + eh2 * evk2;
+MCRT_DEV Plane<R> scat_plane(const R* m, R kx, R ky, R kz, R e1, R e2, R g0, R g1, R g2) {
+  Plane<R> q;
+  q.rx = m[0] * kx + m[3] * ky + m[6] * kz;
+  q.ry = m[1] * kx + m[4] * ky;
+  q.mu = m[2] * kx + m[5] * ky + m[7] * kz;
+  q.pn = m[2] * g0 + m[5] * g1 + m[7] * g2;
+  q.w = m[2] * e1 + m[5] * e2;
+  R r2 = q.rx * q.rx + q.ry * q.ry;
+  if (r2 < R(1e-24)) {  // Ehₖ = -ϕ̂, Evₖ = sign(μ)·θ̂
+    const R u = e1 * m[0] + e2 * m[3], v = e1 * m[1] + e2 * m[4];
+    q.rx = R(1);
+    q.ry = R(0);
+    q.pn = -v;
+    q.w = q.mu > R(0) ? -u : (q.mu < R(0) ? u : R(0));
+    r2 = R(1);
+  }
+  q.inv = M<R>::div(R(1), r2);
+  return q;
 }
 
-// Peel-off Stokes vector of one direction before attenuation: update_I
-// (utils.jl:91-112) with the bin's basis precomputed.
+// Peel-off Stokes vector of one bin before attenuation: update_I
+// (utils.jl:91-112) = R(ψ)·P(μ)·R(-ϕₖ)·I on the scattering plane q.  I is the
+// event's Stokes vector, J = (2·I₂, -2·I₃).  Returns (o0, o1·ρ², o2·ρ², o3):
+// the middle pair still carries the factor ρ² (multiply by q.inv).
 template <typename R, bool MIE>
-MCRT_DEV void peel_I(const R* F, R scale, int imax, const R* m, const R* I, R kx, R ky, R kz,
-                     R eh1, R eh2, R out[4]) {
-  R mu, cpk, spk, cpsi, ehev;
-  frame_part(m, kx, ky, kz, eh1, eh2, mu, cpk, spk, cpsi, ehev);
-  R spsi = -ehev;
+MCRT_DEV void peel_I(const CoopParams<R>& p, const R* F, const R* I, const R* J,
+                     const Plane<R>& q, R o[4]) {
   R P[4];
-  phase_tab<R, MIE>(F, scale, imax, mu, P);
-  R c2 = cpk * cpk - spk * spk, s2 = R(-2) * spk * cpk;
-  R i1 = c2 * I[1] + s2 * I[2];
-  R i2 = -s2 * I[1] + c2 * I[2];
-  R o0 = P[0] * I[0] + P[1] * i1;
-  R o1 = P[1] * I[0] + P[0] * i1;
-  R o2 = P[2] * i2 + P[3] * I[3];
-  R o3 = -P[3] * i2 + P[2] * I[3];
-  R c2p = cpsi * cpsi - spsi * spsi, s2p = R(2) * spsi * cpsi;
-  out[0] = o0;
-  out[1] = c2p * o1 + s2p * o2;
-  out[2] = -s2p * o1 + c2p * o2;
-  out[3] = o3;
+  phase_tab<R, MIE>(p, F, q.mu, P);
+  const R c2 = q.rx * q.rx - q.ry * q.ry;  // ρ²·cos2ϕₖ
+  const R sh = q.rx * q.ry;                // -ρ²·sin2ϕₖ / 2
+  const R i1 = (c2 * I[1] + sh * J[1]) * q.inv;
+  const R i2 = (sh * J[0] + c2 * I[2]) * q.inv;
+  const R a0 = P[0] * I[0] + P[1] * i1;
+  const R a1 = P[1] * I[0] + P[0] * i1;
+  const R a2 = P[2] * i2 + P[3] * I[3];
+  const R a3 = -P[3] * i2 + P[2] * I[3];
+  const R c2p = q.pn * q.pn - q.w * q.w;   // ρ²·cos2ψ
+  const R s2p = R(2) * q.w * q.pn;         // ρ²·sin2ψ
+  o[0] = a0;
+  o[1] = c2p * a1 + s2p * a2;
+  o[2] = -s2p * a1 + c2p * a2;
+  o[3] = a3;
 }
 
-// update_E (utils.jl:170-192) with the basis of kf precomputed.
+// update_E (utils.jl:170-192) on the scattering plane q: E ← Rψ·S(μ)·Rϕₖ·E,
+// returned times ρ² (multiply by q.inv).
 template <typename R, bool MIE>
-MCRT_DEV void peel_E(const R* S, R scale, int imax, const R* m, Cx<R>& E1, Cx<R>& E2, R kx, R ky,
-                     R kz, R eh1, R eh2) {
-  R mu, cpk, spk, cpsi, spsi;
-  frame_part(m, kx, ky, kz, eh1, eh2, mu, cpk, spk, cpsi, spsi);
+MCRT_DEV void peel_E(const CoopParams<R>& p, const R* S, const Plane<R>& q, Cx<R>& E1, Cx<R>& E2) {
   Cx<R> S11, S22;
-  amp_tab<R, MIE>(S, scale, imax, mu, S11, S22);
-  Cx<R> e1 = E1 * cpk + E2 * spk;
-  Cx<R> e2 = (-E1) * spk + E2 * cpk;
-  e1 = S11 * e1;
-  e2 = S22 * e2;
-  E1 = e1 * cpsi + e2 * spsi;
-  E2 = (-e1) * spsi + e2 * cpsi;
+  amp_tab<R, MIE>(p, S, q.mu, S11, S22);
+  const Cx<R> f1 = S11 * (E1 * q.rx + E2 * q.ry);
+  const Cx<R> f2 = S22 * (E2 * q.rx + E1 * -q.ry);
+  E1 = f1 * q.pn + f2 * -q.w;
+  E2 = f1 * q.w + f2 * q.pn;
 }
+
+// Attenuation scale: positions and path lengths of the RT records are stored
+// times kExpK so that the FP32 form evaluates e^{-t} as one exp2.
+template <typename R> struct ExpK;
+template <> struct ExpK<float> {
+  static constexpr float k = 1.4426950408889634f;  // log₂e
+  static MCRT_DEV float exp_neg(float t) { return __builtin_amdgcn_exp2f(-t); }
+};
+template <> struct ExpK<double> {
+  static constexpr double k = 1.0;
+  static MCRT_DEV double exp_neg(double t) { return ::exp(-t); }
+};
 
 // cos/sin of a large FP64 phase: exact FP64 reduction to revolutions, then the
 // R-precision evaluation of the reduced angle.
@@ -343,11 +380,13 @@ template <typename R> struct Slot {
 // thread and all E events of the iteration.  Thread tid owns the bins
 // lb = tid + kThreads·(c0 + i) of the θ-range [jb, jb + nrow) × Nϕ; the escape
 // sum of each event is wave-reduced and added by lane 0 to esc in chunk order.
+// RT record: m[0..7], I[8..11], J[12..13], K·x[14..16], [17] K·τ of the layer
+// (plane) or K²·(τ² - x·x) (sphere), [18] flags; K = ExpK<R>::k.
 template <int CH, typename R, int GEO, bool MIE>
 MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc, double* slab,
                       double* slabc, int E, int tid, int lane, int wave, int jb, int nrow, int nb,
                       int c0, int flag, int slot2) {
-  R bk[CH][7];
+  R bk[CH][kBinW];
   int bidx[CH];
   R acc[CH][4];
 #pragma unroll
@@ -357,42 +396,38 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
     const int j1 = jb + lb % nrow, j2 = lb / nrow;
     const int ob = j1 + p.Nt * j2;
     bidx[i] = valid ? ob : -1;
-    const R* src = p.rt_bin + 8 * (valid ? ob : 0);
+    const R* src = p.rt_bin + kBinStride * (valid ? ob : 0);
 #pragma unroll
-    for (int c = 0; c < 7; ++c) bk[i][c] = src[c];
-    if (!valid) bk[i][5] = R(0);  // zero weight
+    for (int c = 0; c < kBinW; ++c) bk[i][c] = src[c];
+    if (!valid) bk[i][8] = R(0);  // zero weight
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[i][c] = R(0);
   }
-  const R scale = p.scale;
-  const int imax = p.sca.Nt - 1;
-  const R tau0 = p.tau0;
+  const R tau0k = p.tau0k;
   for (int e = 0; e < E; ++e) {
     const R* rec = rtrec + kRtRec * e;
-    const int fl = (int)rec[16];
+    const int fl = (int)rec[18];
     if (!(fl & flag)) continue;
-    R m[8], I[4], pos[4];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) m[c] = rec[c];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) I[c] = rec[8 + c];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) pos[c] = rec[12 + c];
     R part = R(0);
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
+      const Plane<R> q = scat_plane(rec, bk[i][0], bk[i][1], bk[i][2], bk[i][3], bk[i][4],
+                                    bk[i][5], bk[i][6], bk[i][7]);
       R o[4];
-      peel_I<R, MIE>(F, scale, imax, m, I, bk[i][0], bk[i][1], bk[i][2], bk[i][3], bk[i][4], o);
-      R t;
-      if (GEO == 0) {  // distance_to_boundary, plane.jl:97-105 (pos[3]: τ of the layer or ∞)
-        t = bk[i][2] > R(0) ? -pos[2] * bk[i][6] : (pos[2] + pos[3]) * -bk[i][6];
-      } else {         // sphere.jl:69-73 (pos[3] = τ² - x·x)
-        R kx = bk[i][0] * pos[0] + bk[i][1] * pos[1] + bk[i][2] * pos[2];
-        t = -kx + sqrt0(kx * kx + pos[3]);
+      peel_I<R, MIE>(p, F, rec + 8, rec + 12, q, o);
+      R t;  // K·distance_to_boundary
+      if (GEO == 0) {  // plane.jl:97-105
+        t = (bk[i][2] > R(0) ? rec[16] : rec[16] + rec[17]) * -bk[i][9];
+      } else {         // sphere.jl:69-73
+        const R kx = bk[i][0] * rec[14] + bk[i][1] * rec[15] + bk[i][2] * rec[16];
+        t = sqrt_pos(kx * kx + rec[17]) - kx;
       }
-      R w = t <= tau0 ? bk[i][5] * M<R>::exp(-t) : R(0);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) acc[i][c] += o[c] * w;
+      const R w = t <= tau0k ? bk[i][8] * ExpK<R>::exp_neg(t) : R(0);
+      const R wi = w * q.inv;
+      acc[i][0] += o[0] * w;
+      acc[i][1] += o[1] * wi;
+      acc[i][2] += o[2] * wi;
+      acc[i][3] += o[3] * w;
       part += o[0] * w;
     }
     part = wave_sum(part);
@@ -409,18 +444,19 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
 }
 
 // CB bins × events (cpu.jl:83-158): thread tid owns CB bins tid, tid + kThreads, ….
+// CB bin: kf[0..2], Eh[3..4], Eh × kf [5..7], cos2ψ_b [8], sin2ψ_b [9], 1/kf_z [10].
 template <typename R, int GEO, bool MIE>
 MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* cbrec,
                       const double* cbdx, double* slab, double* slabc, int E, int tid) {
   const int nb = p.Ntb * p.Npb;
-  const R scale = p.scale;
-  const int imax = p.sca.Nt - 1;
   const R tau0 = p.tau0;
   const R tau2 = p.tau[0] * p.tau[0];
   for (int lb = tid; lb < nb; lb += kThreads) {
-    const R* bk = p.cb_bin + 8 * lb;
-    const R kfx = bk[0], kfy = bk[1], kfz = bk[2], eh1 = bk[3], eh2 = bk[4];
-    const R c2p = bk[5], s2p = bk[6], ikz = bk[7];
+    R b[kBinW];
+    const R* src = p.cb_bin + kBinStride * lb;
+#pragma unroll
+    for (int c = 0; c < kBinW; ++c) b[c] = src[c];
+    const R kfx = b[0], kfy = b[1], kfz = b[2], ikz = b[10];
     const double* kk = p.cb_kik + 4 * lb;
     const double kikx = kk[0], kiky = kk[1], kikz = kk[2];
     R acc[4] = {R(0), R(0), R(0), R(0)};
@@ -433,25 +469,27 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
       if (GEO == 0) ta = kfz > R(0) ? -xa.z * ikz : (xa.z + cr[30]) * -ikz;
       else {
         R kx = kfx * xa.x + kfy * xa.y + kfz * xa.z;
-        ta = -kx + sqrt0(kx * kx + tau2 - dot(xa, xa));
+        ta = sqrt_pos(kx * kx + tau2 - dot(xa, xa)) - kx;
       }
       if (!(ta <= tau0)) continue;
+      const Plane<R> qa = scat_plane(cr + 8, kfx, kfy, kfz, b[3], b[4], b[5], b[6], b[7]);
       R o[4];
       if (kind == 1) {  // n = 1 (cpu.jl:131-158)
-        peel_I<R, MIE>(F, scale, imax, cr + 8, cr, kfx, kfy, kfz, eh1, eh2, o);
-        R na = M<R>::exp(-ta);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) o[c] = o[c] * na;
+        peel_I<R, MIE>(p, F, cr, cr + 32, qa, o);
+        const R na = M<R>::exp(-ta);
+        const R ni = na * qa.inv;
+        o[0] = o[0] * na; o[1] = o[1] * ni; o[2] = o[2] * ni; o[3] = o[3] * na;
       } else {          // n > 1 (cpu.jl:83-127)
         const V3<R> xb = {cr[27], cr[28], cr[29]};
         Cx<R> a1 = {cr[0], cr[1]}, a2 = {cr[2], cr[3]};
         Cx<R> b1 = {cr[4], cr[5]}, b2 = {cr[6], cr[7]};
-        peel_E<R, MIE>(S, scale, imax, cr + 8, a1, a2, kfx, kfy, kfz, eh1, eh2);
-        peel_E<R, MIE>(S, scale, imax, cr + 16, b1, b2, kfx, kfy, kfz, eh1, eh2);
+        const Plane<R> qb = scat_plane(cr + 16, kfx, kfy, kfz, b[3], b[4], b[5], b[6], b[7]);
+        peel_E<R, MIE>(p, S, qa, a1, a2);
+        peel_E<R, MIE>(p, S, qb, b1, b2);
         const double* dx = cbdx + kCbDx * e;
         const double dphi = p.kl1 * (kikx * dx[0] + kiky * dx[1] + kikz * dx[2]);
         const R tab = (R)dx[3];
-        R na = M<R>::exp(R(-0.5) * ta);
+        const R na = M<R>::exp(R(-0.5) * ta) * qa.inv;
         a1 = a1 * na; a2 = a2 * na;
         R Ia[4];
         I_from_E(a1, a2, Ia);
@@ -459,10 +497,10 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
         if (GEO == 0) tb = kfz > R(0) ? -xb.z * ikz : (xb.z + cr[31]) * -ikz;
         else {
           R kx = kfx * xb.x + kfy * xb.y + kfz * xb.z;
-          tb = -kx + sqrt0(kx * kx + tau2 - dot(xb, xb));
+          tb = sqrt_pos(kx * kx + tau2 - dot(xb, xb)) - kx;
         }
         const bool bok = M<R>::fabs(tb + tab) <= tau0;
-        R nb_ = bok ? M<R>::exp(R(-0.5) * (tb + tab)) : R(0);
+        const R nb_ = bok ? M<R>::exp(R(-0.5) * (tb + tab)) * qb.inv : R(0);
         b1 = b1 * nb_; b2 = b2 * nb_;
         R Ib[4];
         I_from_E(b1, b2, Ib);
@@ -475,7 +513,7 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
 #pragma unroll
         for (int c = 0; c < 4; ++c) o[c] = o[c] * rr;
       }
-      rotate(o, c2p, s2p);
+      rotate(o, b[8], b[9]);
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[c] += o[c];
     }
@@ -502,12 +540,13 @@ __global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __r
   // (every block is a multiple of 16 bytes; must match lds_plan)
   unsigned char* q = lds_raw;
   auto take = [&q](size_t bytes) { unsigned char* r = q; q += (bytes + 15) & ~size_t(15); return r; };
+  // F first: its LDS address is then the lookup offset itself
+  R* ldsF = MIE ? (R*)take(sizeof(R) * 4 * (p.sca.Nt + 1)) : nullptr;
   R* rtrec = (R*)take(sizeof(R) * kRtRec * SLOTS);
   R* cbrec = CB ? (R*)take(sizeof(R) * kCbRec * SLOTS) : nullptr;
   double* cbdx = CB ? (double*)take(sizeof(double) * kCbDx * SLOTS) : nullptr;
   R* esc = (R*)take(sizeof(R) * SLOTS * kWaves * 2);
   int* wcnt = (int*)take(sizeof(int) * 32);
-  R* ldsF = MIE ? (R*)take(sizeof(R) * 4 * (p.sca.Nt + 1)) : nullptr;
   R* ldsC = MIE ? (R*)take(sizeof(R) * (p.sca.rp + 1)) : nullptr;
   R* ldsS = STAGE_S ? (R*)take(sizeof(R) * 4 * (p.sca.Nt + 1)) : nullptr;
 
@@ -749,10 +788,14 @@ __global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __r
         R* rec = rtrec + kRtRec * e_me;
         frame_matrix(sph(s.k), rec);
         rec[8] = s.I[0]; rec[9] = s.I[1]; rec[10] = s.I[2]; rec[11] = s.I[3];
-        rec[12] = s.x.x; rec[13] = s.x.y; rec[14] = s.x.z;
-        if (GEO == 0) rec[15] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
-        else rec[15] = p.tau[0] * p.tau[0] - dot(s.x, s.x);
-        rec[16] = R(near_up | (near_lo << 1));
+        rec[12] = R(2) * s.I[1]; rec[13] = R(-2) * s.I[2];
+        {
+          const R K = ExpK<R>::k;
+          rec[14] = K * s.x.x; rec[15] = K * s.x.y; rec[16] = K * s.x.z;
+          if (GEO == 0) rec[17] = p.isinf[s.idx] ? R(INFINITY) : K * p.tau[s.idx];
+          else rec[17] = K * K * (p.tau[0] * p.tau[0] - dot(s.x, s.x));
+        }
+        rec[18] = R(near_up | (near_lo << 1));
         for (int w = 0; w < kWaves; ++w) {
           esc[(e_me * kWaves + w) * 2] = R(0);
           esc[(e_me * kWaves + w) * 2 + 1] = R(0);
@@ -804,6 +847,7 @@ __global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __r
             frame_matrix(Fr<R>{(R)kas.ct, (R)kas.st, (R)kas.cp, (R)kas.sp}, ma_);
             for (int c = 0; c < 8; ++c) cr[8 + c] = ma_[c];
             cr[0] = s.I[0]; cr[1] = s.I[1]; cr[2] = s.I[2]; cr[3] = s.I[3];
+            cr[32] = R(2) * s.I[1]; cr[33] = R(-2) * s.I[2];
             cr[24] = s.x.x; cr[25] = s.x.y; cr[26] = s.x.z;
             cr[30] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
             cr[39] = R(1);
@@ -859,7 +903,7 @@ __global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __r
         Iref += esc[(e_me * kWaves + w) * 2];
         Itrans += esc[(e_me * kWaves + w) * 2 + 1];
       }
-      const int fl = (int)rtrec[kRtRec * e_me + 16];
+      const int fl = (int)rtrec[kRtRec * e_me + 18];
       for (int rg = 0; rg < p.nranges; ++rg)
         if (fl & (p.range_kind[rg] == 1 ? 1 : 2))
           cnt[kCntRt] += (unsigned long long)(p.range_e[rg] - p.range_b[rg]) * p.Np;
@@ -916,6 +960,31 @@ __global__ void slab_reduce(const double* __restrict__ slabs, int nslab, int64_t
   if (i < 5) A[i] = s;
   else if (i < 5 + nrt) Irt[i - 5] = s;
   else Icb[i - 5 - nrt] = s;
+}
+
+// Standard error of each normalised output T/N by batch means over the
+// workgroup ray ranges: with S_g the raw sum of workgroup g over its n_g rays,
+// v = Σ_g (S_g - n_g·T/N)² / n_g / (G - 1) estimates the per-ray variance and
+// σ = √(v/N).  NaN when G < 2.
+__global__ void slab_sigma(const double* __restrict__ slabs, int nslab, int64_t nout, int64_t N,
+                           double* __restrict__ sig) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nout) return;
+  if (nslab < 2) { sig[i] = NAN; return; }
+  double T = 0.0, c = 0.0;
+  for (int g = 0; g < nslab; ++g) {
+    const double* sl = slabs + (size_t)g * 2 * nout;
+    kahan(&T, &c, sl[i] - sl[nout + i]);
+  }
+  const double mean = T / (double)N;
+  double v = 0.0;
+  for (int g = 0; g < nslab; ++g) {
+    const double* sl = slabs + (size_t)g * 2 * nout;
+    const int64_t ng = N * (g + 1) / nslab - N * g / nslab;  // the kernel's ray range
+    const double d = (sl[i] - sl[nout + i]) - (double)ng * mean;
+    v += d * d / (double)ng;
+  }
+  sig[i] = sqrt(v / (double)(nslab - 1) / (double)N);
 }
 
 // ------------------------------------------------------------------ host
@@ -1059,7 +1128,16 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
   if (rc) return rc;
   fill_sca<double>(P.sca64, s0, alloc, &rc, err);
   if (rc) return rc;
-  P.scale = mie ? (R)((s0->n_theta - 1) / 3.141592653589793) : R(0);
+  {
+    const double sc = mie ? (s0->n_theta - 1) / 3.141592653589793 : 0.0;
+    // Abramowitz & Stegun 4.4.46: acos(a) = √(1-a)·Σ cₖ aᵏ on [0, 1]
+    static const double c[8] = {-0.0012624911, 0.0066700901, -0.0170881256, 0.0308918810,
+                                -0.0501743046, 0.0889789874, -0.2145988016, 1.5707963050};
+    P.scale = (R)sc;
+    for (int k = 0; k < 8; ++k) P.acs[k] = (R)(c[k] * sc);
+    P.pis = (R)(3.141592653589793 * sc);
+    P.tau0k = (R)(cfg->tau0 * (double)ExpK<R>::k);
+  }
   const int Nt = (int)cfg->n_theta, Np = (int)cfg->n_phi;
   const int Ntb = cb ? (int)cfg->n_theta_b : 0, Npb = cb ? (int)cfg->n_phi_b : 0;
   P.Nt = Nt; P.Np = Np; P.Ntb = Ntb; P.Npb = Npb;
@@ -1083,22 +1161,22 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
   }
   HFr kfs = h_sph(P.kf0);
   // RT bin constants: direction (cpu.jl:13), basis (get_basis), inorm, 1/k_z
-  std::vector<R> rtb(8 * (size_t)Nt * Np);
+  std::vector<R> rtb(kBinStride * (size_t)Nt * Np);
   for (int j2 = 0; j2 < Np; ++j2)
     for (int j1 = 0; j1 < Nt; ++j1) {
       double k[3] = {cfg->sin_theta[j1] * cfg->cos_phi[j2], cfg->sin_theta[j1] * cfg->sin_phi[j2],
                      cfg->cos_theta[j1]};
       double Eh[3], Ev[3];
       h_get_basis(k, Eh, Ev);
-      R* b = &rtb[8 * ((size_t)j1 + (size_t)Nt * j2)];
+      R* b = &rtb[kBinStride * ((size_t)j1 + (size_t)Nt * j2)];
       b[0] = (R)k[0]; b[1] = (R)k[1]; b[2] = (R)k[2]; b[3] = (R)Eh[0]; b[4] = (R)Eh[1];
-      b[5] = (R)cfg->inorm[j1];
-      b[6] = (R)(1.0 / k[2]);
-      b[7] = 0;
+      b[5] = (R)(Eh[1] * k[2]); b[6] = (R)(-Eh[0] * k[2]); b[7] = (R)(Eh[0] * k[1] - Eh[1] * k[0]);
+      b[8] = (R)cfg->inorm[j1];
+      b[9] = (R)(1.0 / k[2]);
     }
   // CB bin constants: kf = ray_to_norm(bin, frame(-k₀)), its basis, the final
   // rotation (cpu.jl:118-124), k_i + k_f in FP64
-  std::vector<R> cbb(8 * (size_t)std::max(1, Ntb * Npb));
+  std::vector<R> cbb(kBinStride * (size_t)std::max(1, Ntb * Npb));
   std::vector<double> kik(4 * (size_t)std::max(1, Ntb * Npb));
   for (int j2 = 0; j2 < Npb; ++j2)
     for (int j1 = 0; j1 < Ntb; ++j1) {
@@ -1113,11 +1191,13 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
       double cps = cfg->sin_phi_b[j2] * Ehr[0] - cfg->cos_phi_b[j2] * Ehr[1];
       double sps = -cfg->sin_phi_b[j2] * Evr[0] + cfg->cos_phi_b[j2] * Evr[1];
       const size_t lb = (size_t)j1 + (size_t)Ntb * j2;
-      R* b = &cbb[8 * lb];
+      R* b = &cbb[kBinStride * lb];
       b[0] = (R)kf[0]; b[1] = (R)kf[1]; b[2] = (R)kf[2]; b[3] = (R)Eh[0]; b[4] = (R)Eh[1];
-      b[5] = (R)(cps * cps - sps * sps);
-      b[6] = (R)(2.0 * cps * sps);
-      b[7] = (R)(1.0 / kf[2]);
+      b[5] = (R)(Eh[1] * kf[2]); b[6] = (R)(-Eh[0] * kf[2]);
+      b[7] = (R)(Eh[0] * kf[1] - Eh[1] * kf[0]);
+      b[8] = (R)(cps * cps - sps * sps);
+      b[9] = (R)(2.0 * cps * sps);
+      b[10] = (R)(1.0 / kf[2]);
       kik[4 * lb] = k0[0] + kf[0]; kik[4 * lb + 1] = k0[1] + kf[1]; kik[4 * lb + 2] = k0[2] + kf[2];
       kik[4 * lb + 3] = 0;
     }
@@ -1210,6 +1290,14 @@ hipError_t coop_prepare(CoopPlan& cp, hipStream_t s) {
 
 hipError_t coop_launch(CoopPlan& cp, hipStream_t s) {
   return cp.mixed ? launch_r<float>(cp, s) : launch_r<double>(cp, s);
+}
+
+hipError_t coop_sigma(CoopPlan& cp, hipStream_t s, int64_t N, double* sig) {
+  const int th = 256;
+  const int bl = (int)((cp.nout + th - 1) / th);
+  hipLaunchKernelGGL(slab_sigma, dim3(bl), dim3(th), 0, s, (const double*)cp.slabs, cp.nslab,
+                     cp.nout, N, sig);
+  return hipGetLastError();
 }
 
 hipError_t coop_reduce(CoopPlan& cp, hipStream_t s) {

@@ -77,11 +77,25 @@ def _shape_outputs(cfg: Config, A, Irt, Icb):
 
 
 def simulate(cfg: Config, g: AbstractGeometry, I0, *, use_gpu: bool = True, seed: int = DEFAULT_SEED,
-             precision: str = "fp64", device: int = 0, stats: dict | None = None):
-    """simulate(cfg, g, I₀; use_gpu) — MonteCarloRadiativeTransfer.jl:26-85."""
+             precision: str = "fp64", device: int = 0, stats: dict | None = None,
+             sigma: dict | None = None):
+    """simulate(cfg, g, I₀; use_gpu) — MonteCarloRadiativeTransfer.jl:26-85.
+
+    Returns the reference's 7-tuple (Adt, Aabs, Aref, Atrans, Arem, Irt, Icb), all
+    divided by N.  ``stats`` (dict) receives the exact work counters; ``sigma``
+    (dict) receives the Monte Carlo standard error of every output ("A", "Irt",
+    "Icb", shaped like the outputs) and the number of batches behind it
+    (cooperative forms only)."""
     if not use_gpu:
         raise NotImplementedError(
             "use_gpu=false is the reference's Julia CPU path; this package provides the GPU path")
+    if sigma is not None:
+        plan = Plan(cfg, g, I0, seed=seed, precision=precision, device=device)
+        try:
+            plan.launch()
+            return plan.read(stats=stats, sigma=sigma)
+        finally:
+            plan.close()
     g.validate()
     ctx = get_context(device)
     cs, gs, i0, keep = marshal(cfg, g, I0)
@@ -125,7 +139,7 @@ class Plan:
         _abi.check(self.ctx.lib.mcrt_plan_kernel_ms(self.handle, C.byref(ms)), self.ctx.handle)
         return ms.value
 
-    def read(self, stats: dict | None = None):
+    def read(self, stats: dict | None = None, sigma: dict | None = None):
         cfg = self.cfg
         A = np.zeros(5)
         Irt = np.zeros(4 * cfg.Ntheta * cfg.Nphi)
@@ -135,6 +149,17 @@ class Plan:
                                                _abi.dptr(Icb), C.byref(st)), self.ctx.handle)
         if stats is not None:
             stats.update(st.as_dict())
+        if sigma is not None:
+            sA = np.zeros(5)
+            sIrt = np.zeros_like(Irt)
+            sIcb = np.zeros_like(Icb)
+            nb = C.c_int32()
+            _abi.check(self.ctx.lib.mcrt_plan_read_sigma(self.handle, _abi.dptr(sA), _abi.dptr(sIrt),
+                                                         _abi.dptr(sIcb), C.byref(nb)),
+                       self.ctx.handle)
+            out = _shape_outputs(cfg, sA, sIrt, sIcb[: 4 * cfg.Nthetab * cfg.Nphib])
+            sigma.update({"A": np.asarray(out[:5]), "Irt": out[5], "Icb": out[6],
+                          "batches": nb.value})
         return _shape_outputs(cfg, A, Irt, Icb[: 4 * cfg.Nthetab * cfg.Nphib])
 
     def close(self) -> None:
