@@ -51,6 +51,21 @@ def run_form(cfg, g, seed, I0, form, steps, warmup):
     return dict(wall_s=t1 - t0, kernel_ms=kms, stats=st, out=out)
 
 
+def measured_traffic(workload: str, rays: int, form: str):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summaries (profiles/*.md lines `traffic_bytes_per_launch[workload, N rays,
+    form] = B`), or None when no profile covers this workload, size and form."""
+    import re
+
+    pat = re.compile(r"^traffic_bytes_per_launch\[(\w+), (\d+) rays, (\w+)\] = ([0-9.eE+]+)\s*$")
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            m = pat.match(line.strip())
+            if m and m.group(1) == workload and int(m.group(2)) == rays and m.group(3) == form:
+                return float(m.group(4))
+    return None
+
+
 def copy_calibration(mib: int) -> int:
     """hipMemcpy device→device of mib MiB: one copy kernel that reads and writes
     exactly mib MiB, for scaling the FETCH_SIZE/WRITE_SIZE counters."""
@@ -184,7 +199,8 @@ def main() -> int:
                    "mie_table": getattr(g, "scatterer", None) and getattr(g.scatterer, "Ntheta", None),
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[head],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[head], "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[head],
+                     "traffic": measured_traffic(args.workload, N, head),
                      "kernel_ms": kavg,
                      "flops_per_launch": flops,
                      "units_per_launch": {"rt_evals": st["rt_evals"], "cb_evals": st["cb_evals"],

@@ -203,6 +203,14 @@ int mcrt_simulate(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geometry* ge
 int mcrt_plan_create(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geometry* geom,
                      const double I0[4], uint64_t seed, int32_t precision,
                      mcrt_plan** out);
+/* A plan over rays ray_begin .. ray_begin+N-1 of the stream (seed, instance),
+ * N = cfg->number_of_rays; mcrt_plan_create is ray_begin = 0, instance = 0.
+ * Plans over disjoint ranges of one stream are shards of one larger solve:
+ * their N-weighted mean is that solve's output (up to summation order).  No
+ * reference counterpart (the reference's RNG has no stream position). */
+int mcrt_plan_create_range(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geometry* geom,
+                           const double I0[4], uint64_t seed, int32_t precision,
+                           uint64_t ray_begin, uint32_t instance, mcrt_plan** out);
 int mcrt_plan_launch(mcrt_plan* plan);   /* asynchronous on the context stream */
 int mcrt_plan_wait(mcrt_plan* plan);     /* blocks until the launch finished */
 int mcrt_plan_read(mcrt_plan* plan, double A[5], double* Irt, double* Icb,

@@ -325,7 +325,7 @@ int upload_bins(mcrt_plan* P, const mcrt_config* c, mcrt::BinsDev* B) {
 }
 
 int create_plain(mcrt_plan* P, const mcrt_config* c, const mcrt_geometry* g, const double I0[4],
-                 uint64_t seed) {
+                 uint64_t seed, uint64_t ray0, uint32_t instance) {
   mcrt::PlainParams& p = P->plain;
   int rc = 0;
   int slot[mcrt::kMaxLayers];
@@ -357,7 +357,8 @@ int create_plain(mcrt_plan* P, const mcrt_config* c, const mcrt_geometry* g, con
   for (int i = 0; i < 4; ++i) p.I0[i] = I0[i];
   mcrt::incident(g, p.k0, p.f0);
   p.seed = seed;
-  p.instance = 0;
+  p.ray0 = ray0;
+  p.instance = instance;
   // threads = min(maximum_threads, number_of_rays), blocks of 256 (MonteCarloRadiativeTransfer.jl:39,46)
   int64_t mt = c->maximum_threads > 0 ? c->maximum_threads : 8192;
   int64_t threads = mt < c->number_of_rays ? mt : c->number_of_rays;
@@ -379,9 +380,10 @@ int create_plain(mcrt_plan* P, const mcrt_config* c, const mcrt_geometry* g, con
 
 }  // namespace
 
-extern "C" int mcrt_plan_create(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geometry* geom,
-                                const double I0[4], uint64_t seed, int32_t precision,
-                                mcrt_plan** out) {
+extern "C" int mcrt_plan_create_range(mcrt_ctx* ctx, const mcrt_config* cfg,
+                                      const mcrt_geometry* geom, const double I0[4], uint64_t seed,
+                                      int32_t precision, uint64_t ray_begin, uint32_t instance,
+                                      mcrt_plan** out) {
   if (!ctx) return fail(nullptr, MCRT_ERR_INVALID, "context is NULL");
   if (!out || !I0) return fail(ctx, MCRT_ERR_INVALID, "plan_create: NULL argument");
   *out = nullptr;
@@ -409,10 +411,10 @@ extern "C" int mcrt_plan_create(mcrt_ctx* ctx, const mcrt_config* cfg, const mcr
     return rc;
   }
   if (precision == MCRT_PRECISION_FP64_PLAIN) {
-    rc = create_plain(P, cfg, geom, I0, seed);
+    rc = create_plain(P, cfg, geom, I0, seed, ray_begin, instance);
   } else {
     rc = mcrt::coop_create(P->coop, ctx->stream, ctx->cu_count, ctx->lds_per_block, cfg, geom, I0, seed,
-                           precision == MCRT_PRECISION_MIXED,
+                           ray_begin, instance, precision == MCRT_PRECISION_MIXED,
                            [P](size_t n, hipError_t* err) { return P->alloc(n, err); },
                            P->dA, P->dIrt, P->dIcb, P->dcnt, &ctx->err);
     if (rc) g_thread_err = ctx->err;
@@ -428,6 +430,12 @@ extern "C" int mcrt_plan_create(mcrt_ctx* ctx, const mcrt_config* cfg, const mcr
   }
   *out = P;
   return MCRT_OK;
+}
+
+extern "C" int mcrt_plan_create(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geometry* geom,
+                                const double I0[4], uint64_t seed, int32_t precision,
+                                mcrt_plan** out) {
+  return mcrt_plan_create_range(ctx, cfg, geom, I0, seed, precision, 0, 0, out);
 }
 
 extern "C" int mcrt_plan_launch(mcrt_plan* P) {

@@ -36,7 +36,8 @@ using DevAlloc = std::function<void*(size_t, hipError_t*)>;
 // Validate, stage tables and parameters in device memory.  Returns an MCRT_*
 // status and writes a message to *err on failure.
 int coop_create(CoopPlan& cp, hipStream_t stream, int cu_count, size_t lds_max, const mcrt_config* cfg,
-                const mcrt_geometry* geom, const double I0[4], uint64_t seed, bool mixed,
+                const mcrt_geometry* geom, const double I0[4], uint64_t seed, uint64_t ray0,
+                uint32_t instance, bool mixed,
                 const DevAlloc& alloc, double* dA, double* dIrt, double* dIcb,
                 unsigned long long* dcnt, std::string* err);
 

@@ -94,6 +94,7 @@ template <typename R> struct CoopParams {
   double norm0;   // |E_i|
   double kf0[3];  // -k₀ (exact backscatter)
   uint64_t seed;
+  uint64_t ray0;  // stream number of the first ray
   uint32_t instance;
   void* slot_mem;         // [blocks][SLOTS] Slot<R>: per-slot ray state
   double* slabs;          // [blocks][2][nout]: Kahan sum, compensation
@@ -509,7 +510,8 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
         Cx<R> ep = {cph, sph_};
         Cx<R> e1 = a1 * ep + b1, e2 = a2 * ep + b2;
         I_from_E(e1, e2, o);
-        R rr = M<R>::div(Ia[0], Ia[0] + Ib[0]);
+        const R den = Ia[0] + Ib[0];
+        const R rr = den > R(0) ? M<R>::div(Ia[0], den) : R(0);  // 0 where both fields vanish
 #pragma unroll
         for (int c = 0; c < 4; ++c) o[c] = o[c] * rr;
       }
@@ -601,7 +603,7 @@ __global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __r
         if (owner && !s.live && next_ray + off < re) {
           // a new ray: initial_x, direct transmission, initial_propagation
           s.live = 1;
-          s.ray = (uint64_t)(next_ray + off);
+          s.ray = p.ray0 + (uint64_t)(next_ray + off);
           Rng rng{p.seed, s.ray, p.instance, 0};
 #pragma unroll
           for (int c = 0; c < 4; ++c) s.I[c] = p.I0[c];
@@ -823,8 +825,11 @@ __global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __r
             Cx<double> Eb1 = v1 * sc, Eb2 = v2 * sc;
             Cx<double> Eaf1 = s.Ea1, Eaf2 = s.Ea2;
             update_E(p.sca64, Eaf1, Eaf2, kf0, kas);
-            double raa = sqrt(Iaf[0] / (Eaf1.re * Eaf1.re + Eaf1.im * Eaf1.im + Eaf2.re * Eaf2.re +
-                                        Eaf2.im * Eaf2.im));
+            // raa (cpu.jl:68-69); zero where rounding leaves Iaf₁ ≤ 0 or the
+            // field is zero (the reference would raise DomainError or give NaN)
+            const double eaf2 = Eaf1.re * Eaf1.re + Eaf1.im * Eaf1.im + Eaf2.re * Eaf2.re +
+                                Eaf2.im * Eaf2.im;
+            const double raa = (Iaf[0] > 0 && eaf2 > 0) ? sqrt(Iaf[0] / eaf2) : 0.0;
             cr[0] = (R)(s.Ea1.re * raa); cr[1] = (R)(s.Ea1.im * raa);
             cr[2] = (R)(s.Ea2.re * raa); cr[3] = (R)(s.Ea2.im * raa);
             cr[4] = (R)(Eb1.re * raa); cr[5] = (R)(Eb1.im * raa);
@@ -1089,7 +1094,8 @@ template <typename R> size_t lds_plan(bool mie, bool cb, int nt, int rp) {
 
 template <typename R>
 int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* cfg,
-                const mcrt_geometry* g, const double I0[4], uint64_t seed, const DevAlloc& alloc,
+                const mcrt_geometry* g, const double I0[4], uint64_t seed, uint64_t ray0,
+                uint32_t instance, const DevAlloc& alloc,
                 unsigned long long* dcnt, std::string* err) {
   CoopParams<R> P;
   std::memset(&P, 0, sizeof P);
@@ -1217,7 +1223,8 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
   P.cb_kik = (const double*)up(kik.data(), sizeof(double) * kik.size());
   if (rc) return rc;
   P.seed = seed;
-  P.instance = 0;
+  P.ray0 = ray0;
+  P.instance = instance;
   P.nrt = 4 * (int64_t)Nt * Np;
   P.ncb = 4 * (int64_t)cfg->n_theta_b * cfg->n_phi_b;
   P.nout = 5 + P.nrt + P.ncb;
@@ -1273,15 +1280,18 @@ hipError_t launch_r(const CoopPlan& cp, hipStream_t s) {
 }  // namespace
 
 int coop_create(CoopPlan& cp, hipStream_t, int cu_count, size_t lds_max, const mcrt_config* cfg,
-                const mcrt_geometry* geom, const double I0[4], uint64_t seed, bool mixed,
+                const mcrt_geometry* geom, const double I0[4], uint64_t seed, uint64_t ray0,
+                uint32_t instance, bool mixed,
                 const DevAlloc& alloc, double* dA, double* dIrt, double* dIcb,
                 unsigned long long* dcnt, std::string* err) {
   cp.mixed = mixed;
   cp.dA = dA; cp.dIrt = dIrt; cp.dIcb = dIcb;
   if (cu_count <= 0) cu_count = 256;
   if (lds_max == 0) lds_max = 160 * 1024;
-  return mixed ? create_impl<float>(cp, cu_count, lds_max, cfg, geom, I0, seed, alloc, dcnt, err)
-               : create_impl<double>(cp, cu_count, lds_max, cfg, geom, I0, seed, alloc, dcnt, err);
+  return mixed ? create_impl<float>(cp, cu_count, lds_max, cfg, geom, I0, seed, ray0, instance, alloc,
+                                    dcnt, err)
+               : create_impl<double>(cp, cu_count, lds_max, cfg, geom, I0, seed, ray0, instance, alloc,
+                                     dcnt, err);
 }
 
 hipError_t coop_prepare(CoopPlan& cp, hipStream_t s) {

@@ -57,6 +57,7 @@ struct PlainParams {
   double k0[3];
   double f0[4];  // (cosθ₀, sinθ₀, cosϕ₀, sinϕ₀) of incidence_angle(g)
   uint64_t seed;
+  uint64_t ray0;  // stream number of the first ray
   uint32_t instance;
   double* kpath;      // [threads][maxscat][3]  (column per thread, as CuMatrix(maxscat, threads))
   double* xpath;

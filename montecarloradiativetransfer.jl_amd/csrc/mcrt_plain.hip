@@ -129,8 +129,10 @@ MCRT_DEV void collect_cb(const Ctx& c, const R I[4], V3<R> k, V3<R> x, int idx, 
     Eb1 = Eb1 * sc; Eb2 = Eb2 * sc;
     Cx<R> Eaf1 = Ea1, Eaf2 = Ea2;
     update_E(sca, Eaf1, Eaf2, kf0, kas);
-    R raa = sqrt(Iaf[0] / (Eaf1.re * Eaf1.re + Eaf1.im * Eaf1.im + Eaf2.re * Eaf2.re +
-                           Eaf2.im * Eaf2.im));
+    // raa (cpu.jl:68-69); zero where rounding leaves Iaf₁ ≤ 0 or the field is
+    // zero (the reference would raise DomainError or propagate NaN)
+    const R eaf2 = Eaf1.re * Eaf1.re + Eaf1.im * Eaf1.im + Eaf2.re * Eaf2.re + Eaf2.im * Eaf2.im;
+    R raa = (Iaf[0] > 0 && eaf2 > 0) ? sqrt(Iaf[0] / eaf2) : R(0);
     Ea1 = Ea1 * raa; Ea2 = Ea2 * raa;
     Eb1 = Eb1 * raa; Eb2 = Eb2 * raa;
     V3<R> dx = x - xb;
@@ -165,7 +167,8 @@ MCRT_DEV void collect_cb(const Ctx& c, const R I[4], V3<R> k, V3<R> x, int idx, 
           Cx<R> e1 = a1 * ep + b1, e2 = a2 * ep + b2;
           R Iab[4];
           I_from_E(e1, e2, Iab);
-          R rr = Ia[0] / (Ia[0] + Ib[0]);
+          const R den = Ia[0] + Ib[0];
+          R rr = den > 0 ? Ia[0] / den : R(0);  // 0 where both fields vanish
           for (int q = 0; q < 4; ++q) Iab[q] = Iab[q] * rr;
           V3<R> Ehr = norm_to_ray(Eh, kfs), Evr = norm_to_ray(Ev, kfs);
           R cpsi = b.spb[j2] * Ehr.x - b.cpb[j2] * Ehr.y;
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(256) void plain_kernel(const PlainParams* __restric
   const bool can_dt = (g.kind == 1) || (g.nl == 1 && !g.isinf[0]);
   unsigned long long cnt[kCntN] = {};
   for (int64_t ray = nid; ray < p.N; ray += stride) {
-    Rng rng{p.seed, (uint64_t)ray, p.instance, 0};
+    Rng rng{p.seed, p.ray0 + (uint64_t)ray, p.instance, 0};
     R I[4] = {p.I0[0], p.I0[1], p.I0[2], p.I0[3]};
     V3<R> k = {p.k0[0], p.k0[1], p.k0[2]};
     V3<R> x;

@@ -146,6 +146,9 @@ SIGNATURES = {
     "mcrt_plan_wait": (C.c_int, [_vp]),
     "mcrt_plan_read": (C.c_int, [_vp, _dp, _dp, _dp, C.POINTER(mcrt_stats)]),
     "mcrt_plan_kernel_ms": (C.c_int, [_vp, _dp]),
+    "mcrt_plan_create_range": (C.c_int, [_vp, C.POINTER(mcrt_config), C.POINTER(mcrt_geometry), _dp,
+                                         C.c_uint64, C.c_int32, C.c_uint64, C.c_uint32,
+                                         C.POINTER(_vp)]),
     "mcrt_plan_read_sigma": (C.c_int, [_vp, _dp, _dp, _dp, C.POINTER(C.c_int32)]),
     "mcrt_plan_destroy": (None, [_vp]),
 }

@@ -78,19 +78,21 @@ def _shape_outputs(cfg: Config, A, Irt, Icb):
 
 def simulate(cfg: Config, g: AbstractGeometry, I0, *, use_gpu: bool = True, seed: int = DEFAULT_SEED,
              precision: str = "fp64", device: int = 0, stats: dict | None = None,
-             sigma: dict | None = None):
+             sigma: dict | None = None, ray_begin: int = 0, instance: int = 0):
     """simulate(cfg, g, I₀; use_gpu) — MonteCarloRadiativeTransfer.jl:26-85.
 
     Returns the reference's 7-tuple (Adt, Aabs, Aref, Atrans, Arem, Irt, Icb), all
     divided by N.  ``stats`` (dict) receives the exact work counters; ``sigma``
     (dict) receives the Monte Carlo standard error of every output ("A", "Irt",
     "Icb", shaped like the outputs) and the number of batches behind it
-    (cooperative forms only)."""
+    (cooperative forms only).  ``ray_begin``/``instance`` select the rays
+    ray_begin .. ray_begin+N-1 of the Philox stream (seed, instance)."""
     if not use_gpu:
         raise NotImplementedError(
             "use_gpu=false is the reference's Julia CPU path; this package provides the GPU path")
-    if sigma is not None:
-        plan = Plan(cfg, g, I0, seed=seed, precision=precision, device=device)
+    if sigma is not None or ray_begin or instance:
+        plan = Plan(cfg, g, I0, seed=seed, precision=precision, device=device,
+                    ray_begin=ray_begin, instance=instance)
         try:
             plan.launch()
             return plan.read(stats=stats, sigma=sigma)
@@ -116,15 +118,18 @@ class Plan:
     """A resident solve: inputs staged in HBM once, launched any number of times."""
 
     def __init__(self, cfg: Config, g: AbstractGeometry, I0, *, seed: int = DEFAULT_SEED,
-                 precision: str = "fp64", device: int = 0):
+                 precision: str = "fp64", device: int = 0, ray_begin: int = 0, instance: int = 0):
         g.validate()
         self.cfg = cfg
         self.ctx = get_context(device)
         cs, gs, i0, keep = marshal(cfg, g, I0)
         h = C.c_void_p()
-        rc = self.ctx.lib.mcrt_plan_create(self.ctx.handle, C.byref(cs), C.byref(gs), _abi.dptr(i0),
-                                           C.c_uint64(seed & (2**64 - 1)),
-                                           _abi.PRECISIONS[precision], C.byref(h))
+        if ray_begin < 0 or not 0 <= instance < 2**32:
+            raise ValueError("ray_begin must be >= 0 and instance a 32-bit unsigned integer")
+        rc = self.ctx.lib.mcrt_plan_create_range(self.ctx.handle, C.byref(cs), C.byref(gs),
+                                                 _abi.dptr(i0), C.c_uint64(seed & (2**64 - 1)),
+                                                 _abi.PRECISIONS[precision], C.c_uint64(ray_begin),
+                                                 C.c_uint32(instance), C.byref(h))
         _abi.check(rc, self.ctx.handle)
         self.handle = h
 

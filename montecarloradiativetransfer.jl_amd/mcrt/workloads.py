@@ -6,8 +6,10 @@
                         λ=0.55 µm (x≈5.712), m=1.5+0.001i, ν=0.1, F/S tabulated on
                         1801 angles (0.1°), random_points=1000; RT 180 × 1° polar
                         bins × 48 azimuths; CB θb = 0:0.05°:5° (101) × 8; seed 7
-The ray counts of the full workloads are 1e6 and 1e7; tests use fewer rays of
-the same configuration.
+  plane_rayleigh_cb     plane_rayleigh_tau1 with CB on (default θb grid), 1e9 rays:
+                        the mixed-vs-FP64 precision case
+The ray counts of the full workloads are 1e6, 1e7 and 1e9; tests use fewer rays
+of the same configuration.
 """
 from __future__ import annotations
 
@@ -61,4 +63,11 @@ def sphere_mie_cb(n_rays: int = 10**7, *, Ntheta_table: int = 1801, Nphib: int =
     return cfg, g, 7
 
 
-WORKLOADS = {"plane_rayleigh_tau1": plane_rayleigh_tau1, "sphere_mie_cb": sphere_mie_cb}
+def plane_rayleigh_cb(n_rays: int = 10**9):
+    """The plane slab of plane_rayleigh_tau1 with CB on (the reference's default
+    θb grid of 51 angles × 8 azimuths, kl = 1): the mixed-vs-FP64 precision case."""
+    return plane_rayleigh_tau1(n_rays, compute_cb=True)
+
+
+WORKLOADS = {"plane_rayleigh_tau1": plane_rayleigh_tau1, "sphere_mie_cb": sphere_mie_cb,
+             "plane_rayleigh_cb": plane_rayleigh_cb}

@@ -846,8 +846,12 @@ static void collect_cb(const tctx_t* T, double* Icb, const double I[4], const do
     Eb[1] = cscale(Eb[1], sc);
     or_cplx Eaf[2];
     or_update_E(sca, Ea, kf0, kas, Eaf, Eh, Ev);
-    double raa = sqrt(Iaf[0] / (Eaf[0].re * Eaf[0].re + Eaf[0].im * Eaf[0].im +
-                                Eaf[1].re * Eaf[1].re + Eaf[1].im * Eaf[1].im));
+    /* raa = sqrt(Iaf[1]/|Eaf|^2) (cpu.jl:68-69).  Where rounding leaves
+     * Iaf[1] <= 0 or the field is zero, the reference would raise DomainError
+     * or propagate NaN; here the event's CB contribution is zero. */
+    double eaf2 = Eaf[0].re * Eaf[0].re + Eaf[0].im * Eaf[0].im +
+                  Eaf[1].re * Eaf[1].re + Eaf[1].im * Eaf[1].im;
+    double raa = (Iaf[0] > 0 && eaf2 > 0) ? sqrt(Iaf[0] / eaf2) : 0.0;
     Ea[0] = cscale(Ea[0], raa); Ea[1] = cscale(Ea[1], raa);
     Eb[0] = cscale(Eb[0], raa); Eb[1] = cscale(Eb[1], raa);
     double kl = G->sca[0]->kl; /* mean_free_path(g, 1) */
@@ -887,7 +891,9 @@ static void collect_cb(const tctx_t* T, double* Icb, const double I[4], const do
           or_cplx Eab[2] = {cadd(cmul(Eaf2[0], ep), Ebf2[0]), cadd(cmul(Eaf2[1], ep), Ebf2[1])};
           double Iab[4];
           or_I_from_E(Eab, Iab);
-          double rr = Iaf2[0] / (Iaf2[0] + Ibf[0]);
+          /* Iaf/(Iaf + Ibf) (cpu.jl:114); 0 where both fields vanish (0/0) */
+          double den = Iaf2[0] + Ibf[0];
+          double rr = den > 0 ? Iaf2[0] / den : 0.0;
           for (int c = 0; c < 4; ++c) Iab[c] = Iab[c] * rr;
           double Ehr[3], Evr[3];
           or_norm_to_ray(Eh2, kfs, Ehr);
