@@ -230,6 +230,99 @@ void mcrt_plan_destroy(mcrt_plan* plan);
 int mcrt_plan_read_sigma(mcrt_plan* plan, double A_sigma[5], double* Irt_sigma,
                          double* Icb_sigma, int32_t* batches);
 
+/* ---- batched solves ------------------------------------------------------- */
+
+/* One solve of a batch: the arguments of simulate plus its stream position. */
+typedef struct mcrt_request {
+  const mcrt_config* cfg;
+  const mcrt_geometry* geom;
+  double I0[4];
+  uint64_t seed;
+  uint64_t ray_begin;   /* rays ray_begin .. ray_begin+N-1 of the stream */
+  uint32_t instance;
+  int32_t precision;    /* MCRT_PRECISION_FP64 or MCRT_PRECISION_MIXED */
+  int64_t chunk_rays;   /* rays per work chunk; 0: the default for N */
+} mcrt_request;
+
+typedef struct mcrt_batch mcrt_batch;
+
+/* Many solves (any mix of plane/sphere, Rayleigh/Mie, CB on/off, sizes) in
+ * one persistent launch per precision: every solve is cut into chunks of
+ * chunk_rays rays, the chunks of all solves form one queue ordered by their
+ * estimated work, and each workgroup of the launch takes chunks until the
+ * queue is empty.  A chunk's rays never share an iteration with another
+ * chunk, so each solve gets the same result alone and in any batch, up to the
+ * order of FP64 additions.  The sequence (clear, launch, reduce) is captured
+ * in a HIP graph on the first mcrt_batch_launch and replayed afterwards.  The
+ * request structs and what they point to are read during create only.  No
+ * reference counterpart (the reference runs one simulate per call). */
+int mcrt_batch_create(mcrt_ctx* ctx, int32_t n, const mcrt_request* reqs, mcrt_batch** out);
+int mcrt_batch_launch(mcrt_batch* batch);  /* asynchronous on the context stream */
+int mcrt_batch_wait(mcrt_batch* batch);
+/* Results of solve i of the last launch, as mcrt_plan_read / _read_sigma. */
+int mcrt_batch_read(mcrt_batch* batch, int32_t i, double A[5], double* Irt, double* Icb,
+                    double A_sigma[5], double* Irt_sigma, double* Icb_sigma, mcrt_stats* stats);
+/* Device time (ms) of the last launch: clear, transport and reduction. */
+int mcrt_batch_kernel_ms(mcrt_batch* batch, double* ms);
+/* Chunks in the queue and device bytes held by the batch. */
+int mcrt_batch_info(mcrt_batch* batch, int64_t* chunks, int64_t* device_bytes);
+void mcrt_batch_destroy(mcrt_batch* batch);
+
+/* ---- serving -------------------------------------------------------------- */
+
+/* Ticket states. */
+enum {
+  MCRT_TICKET_QUEUED = 0,     /* waiting for device memory */
+  MCRT_TICKET_RUNNING = 1,    /* admitted: its chunks are being launched */
+  MCRT_TICKET_DONE = 2,
+  MCRT_TICKET_CANCELLED = 3,
+  MCRT_TICKET_FAILED = 4
+};
+
+typedef struct mcrt_server_stats {
+  uint64_t submitted, completed, cancelled, failed;
+  uint64_t launches;          /* persistent launches issued */
+  uint64_t chunks;            /* chunks in those launches */
+  uint64_t rays;              /* rays of the completed solves */
+  uint64_t queued, running;   /* tickets now waiting / admitted */
+  int64_t device_bytes_in_use, device_budget;
+  int64_t table_cache_bytes;
+  uint64_t table_hits, table_misses;
+  double busy_ms;             /* wall time with at least one launch in flight */
+} mcrt_server_stats;
+
+typedef struct mcrt_server mcrt_server;
+
+/* A request loop on its own host thread and two HIP streams of the context's
+ * device.  Submitted solves wait until their device memory (bins, batch
+ * slabs, parameters) fits device_budget_bytes (<= 0: half of the free device
+ * memory); Mie tables are kept resident across requests in a cache keyed by
+ * their content (table_cache_bytes, <= 0: 1 GiB).  Admitted solves of one
+ * precision share each persistent launch (continuous batching: a solve that
+ * arrives while others run joins the next launch, queued on the device behind
+ * the current one), and every solve's result is the same alone and in the mix
+ * up to the order of FP64 additions.  No reference counterpart. */
+int mcrt_server_create(mcrt_ctx* ctx, int64_t device_budget_bytes, int64_t table_cache_bytes,
+                       mcrt_server** out);
+/* Queue a solve (the request and what it points to are copied). */
+int mcrt_submit(mcrt_server* srv, const mcrt_request* req, uint64_t* ticket);
+int mcrt_poll(mcrt_server* srv, uint64_t ticket, int32_t* state);
+/* Block until the ticket is done, cancelled or failed (timeout_ms < 0: no
+ * limit); -6 (not ready) on timeout. */
+int mcrt_wait(mcrt_server* srv, uint64_t ticket, double timeout_ms, int32_t* state);
+/* Results of a finished ticket (layouts and normalisation of mcrt_plan_read,
+ * sigma as mcrt_plan_read_sigma; stats->total_ms = latency from submit), then
+ * forget the ticket.  -5 for a cancelled ticket, the failure's status for a
+ * failed one. */
+int mcrt_result(mcrt_server* srv, uint64_t ticket, double A[5], double* Irt, double* Icb,
+                double A_sigma[5], double* Irt_sigma, double* Icb_sigma, mcrt_stats* stats,
+                double* latency_ms);
+/* Stop a queued or running ticket (no effect once its last chunk launched). */
+int mcrt_cancel(mcrt_server* srv, uint64_t ticket);
+int mcrt_server_stats_get(mcrt_server* srv, mcrt_server_stats* out);
+/* Cancel what is left, wait for the device, free everything. */
+void mcrt_server_destroy(mcrt_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,7 +18,8 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libmcrt_hip.so"
-SOURCES = ["mcrt_tables.cpp", "mcrt_api.cpp", "mcrt_plain.hip", "mcrt_coop.hip"]
+SOURCES = ["mcrt_tables.cpp", "mcrt_api.cpp", "mcrt_batch.cpp", "mcrt_serve.cpp", "mcrt_plain.hip",
+           "mcrt_coop.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [
@@ -58,7 +59,8 @@ def build_library(verbose: bool = False, jobs: int = 4) -> Path:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)], verbose)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+          "-lpthread"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -16,14 +16,6 @@
 #include "mcrt_internal.h"
 #include "mcrt_kernels.h"
 
-struct mcrt_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  int cu_count = 0;
-  size_t lds_per_block = 0;
-};
-
 namespace {
 
 thread_local std::string g_thread_err;
@@ -180,6 +172,10 @@ int validate_scatterer(mcrt_ctx* ctx, const mcrt_scatterer* s) {
   return MCRT_OK;
 }
 
+}  // namespace
+
+namespace mcrt {
+
 // validate(p::PlaneGeometry), plane.jl:52-62; validate(s::SphereGeometry), sphere.jl:35-37.
 int validate_geometry(mcrt_ctx* ctx, const mcrt_geometry* g) {
   if (!g) return fail(ctx, MCRT_ERR_INVALID, "geometry is NULL");
@@ -219,6 +215,10 @@ int validate_config(mcrt_ctx* ctx, const mcrt_config* c) {
     return fail(ctx, MCRT_ERR_INVALID, "compute_cb needs the CB angle vectors");
   return MCRT_OK;
 }
+
+}  // namespace mcrt
+
+namespace {
 
 template <typename T>
 T* upload(mcrt_plan* P, const T* host, size_t n, int* rc) {
@@ -388,8 +388,8 @@ extern "C" int mcrt_plan_create_range(mcrt_ctx* ctx, const mcrt_config* cfg,
   if (!out || !I0) return fail(ctx, MCRT_ERR_INVALID, "plan_create: NULL argument");
   *out = nullptr;
   int rc;
-  if ((rc = validate_config(ctx, cfg))) return rc;
-  if ((rc = validate_geometry(ctx, geom))) return rc;
+  if ((rc = mcrt::validate_config(ctx, cfg))) return rc;
+  if ((rc = mcrt::validate_geometry(ctx, geom))) return rc;
   if (precision < MCRT_PRECISION_FP64_PLAIN || precision > MCRT_PRECISION_MIXED)
     return fail(ctx, MCRT_ERR_INVALID, "unknown precision %d", precision);
   HIP_TRY(ctx, hipSetDevice(ctx->device));

@@ -6,6 +6,7 @@
 
 #include <functional>
 #include <string>
+#include <vector>
 
 #include "../../include/mcrt.h"
 #include "mcrt_kernels.h"
@@ -32,6 +33,67 @@ struct CoopPlan {
 };
 
 using DevAlloc = std::function<void*(size_t, hipError_t*)>;
+
+// A ray range [rb, re) of one solve of a batched launch; the sums go to slab
+// (one of the solve's batch slabs) with FP64 atomics.
+struct BatchChunk {
+  const void* params;  // the solve's parameter block (device)
+  int64_t rb, re;
+  double* slab;
+};
+
+// Device copies of scatterer tables keyed by their content, shared by the
+// solves that use them: reference counted, and unreferenced tables are freed,
+// least recently used first, when the cache holds more than its budget.
+struct TableCache;
+TableCache* table_cache_create(size_t budget_bytes);
+void table_cache_destroy(TableCache* c);
+void table_cache_release(TableCache* c, const std::vector<const void*>& refs);
+void table_cache_stats(TableCache* c, size_t* bytes, uint64_t* hits, uint64_t* misses);
+
+// One solve of a batched launch: parameters, tables and bins in device memory,
+// its N rays split into chunks of chunk_rays, chunk k summed into batch slab
+// k mod nbatch.
+struct SolveDev {
+  bool mixed = false;
+  int geo_kind = 0;
+  bool mie = false, cb = false;
+  int64_t N = 0, nout = 0, nrt = 0, ncb = 0;
+  size_t lds_bytes = 0;
+  void* params_dev = nullptr;
+  int64_t chunk_rays = 0, nchunks = 0;
+  int nbatch = 0;
+  double* slabs = nullptr;                 // nbatch × nout raw sums
+  int64_t* batch_rays = nullptr;           // nbatch (device)
+  unsigned long long* counters = nullptr;  // kCntN
+  double* out = nullptr;                   // nout raw sums: A[5], Irt, Icb
+  double* sig = nullptr;                   // nout standard errors of out/N
+  double cost = 0.0;                       // relative work of one chunk
+  std::vector<const void*> table_refs;     // cached tables in use (TableCache)
+};
+
+// Rays per chunk when the caller does not choose: between 512 and 8192, at
+// least two chunks when N ≥ 2.  A function of N alone, so that a solve is cut
+// the same way alone and in any batch.
+int64_t default_chunk_rays(int64_t N);
+// Tables go to memory from alloc, or are shared through cache when given.
+int solve_create(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, const mcrt_geometry* geom,
+                 const double I0[4], uint64_t seed, uint64_t ray0, uint32_t instance, bool mixed,
+                 int64_t chunk_rays, const DevAlloc& alloc, std::string* err,
+                 TableCache* cache = nullptr);
+// Device bytes of a solve apart from cached tables (for admission).
+size_t solve_bytes_estimate(const mcrt_config* cfg, bool mixed, int64_t chunk_rays);
+void solve_chunk(const SolveDev& sd, int64_t k, BatchChunk* out);
+// Zero the batch slabs and counters (kernels, so the sequence can be captured).
+hipError_t solve_prepare(const SolveDev& sd, hipStream_t stream);
+// Reduce the batch slabs into out and sig.
+hipError_t solve_finish(const SolveDev& sd, hipStream_t stream);
+size_t batch_slot_bytes(bool mixed, int blocks);
+hipError_t batch_set_lds(bool mixed, size_t lds_bytes);
+// One persistent launch over nchunks chunks (dnext: a device counter it zeroes
+// first); blocks workgroups, each with its slot state in slot_mem.
+hipError_t batch_launch(bool mixed, const BatchChunk* dchunks, int nchunks, unsigned int* dnext,
+                        void* slot_mem, int blocks, size_t lds_bytes, hipStream_t stream);
 
 // Validate, stage tables and parameters in device memory.  Returns an MCRT_*
 // status and writes a message to *err on failure.

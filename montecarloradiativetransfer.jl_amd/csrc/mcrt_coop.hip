@@ -29,6 +29,8 @@
 //
 // R is the transport / bin-evaluation type (double: FP64 form, float: mixed).
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -383,7 +385,7 @@ template <typename R> struct Slot {
 // sum of each event is wave-reduced and added by lane 0 to esc in chunk order.
 // RT record: m[0..7], I[8..11], J[12..13], K·x[14..16], [17] K·τ of the layer
 // (plane) or K²·(τ² - x·x) (sphere), [18] flags; K = ExpK<R>::k.
-template <int CH, typename R, int GEO, bool MIE>
+template <int CH, typename R, int GEO, bool MIE, bool ATOMIC>
 MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc, double* slab,
                       double* slabc, int E, int tid, int lane, int wave, int jb, int nrow, int nb,
                       int c0, int flag, int slot2) {
@@ -434,19 +436,23 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
     part = wave_sum(part);
     if (lane == 0) esc[(e * kWaves + wave) * 2 + slot2] += part;
   }
-  // flush into the workgroup slab (bins are owned by one thread: no races)
+  // flush into the slab: Kahan into a workgroup-private slab (bins are owned
+  // by one thread: no races), or FP64 atomics into a slab shared by chunks
 #pragma unroll
   for (int i = 0; i < CH; ++i) {
     if (bidx[i] < 0) continue;
     const int64_t o = 5 + 4 * (int64_t)bidx[i];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) kahan(slab + o + c, slabc + o + c, (double)acc[i][c]);
+    for (int c = 0; c < 4; ++c) {
+      if (ATOMIC) unsafeAtomicAdd(slab + o + c, (double)acc[i][c]);
+      else kahan(slab + o + c, slabc + o + c, (double)acc[i][c]);
+    }
   }
 }
 
 // CB bins × events (cpu.jl:83-158): thread tid owns CB bins tid, tid + kThreads, ….
 // CB bin: kf[0..2], Eh[3..4], Eh × kf [5..7], cos2ψ_b [8], sin2ψ_b [9], 1/kf_z [10].
-template <typename R, int GEO, bool MIE>
+template <typename R, int GEO, bool MIE, bool ATOMIC>
 MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* cbrec,
                       const double* cbdx, double* slab, double* slabc, int E, int tid) {
   const int nb = p.Ntb * p.Npb;
@@ -521,65 +527,96 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
     }
     const int64_t ob = 5 + p.nrt + 4 * (int64_t)lb;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) kahan(slab + ob + c, slabc + ob + c, (double)acc[c]);
+    for (int c = 0; c < 4; ++c) {
+      if (ATOMIC) unsafeAtomicAdd(slab + ob + c, (double)acc[c]);
+      else kahan(slab + ob + c, slabc + ob + c, (double)acc[c]);
+    }
   }
 }
 
-template <typename R, int GEO, bool MIE, bool CB>
-__global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __restrict__ pp) {
+// LDS blocks of one workgroup for the solve p; every block is a multiple of 16
+// bytes and lds_plan computes the same total.  F first: its LDS address is then
+// the lookup offset itself.
+template <typename R> struct Lds {
+  R* F;
+  R* rtrec;
+  R* cbrec;
+  double* cbdx;
+  R* esc;
+  int* wcnt;
+  R* C;
+  R* S;
+};
+template <typename R, bool MIE, bool CB>
+MCRT_DEV Lds<R> lds_layout(const CoopParams<R>& p, unsigned char* q) {
+  constexpr int SLOTS = Shape<R>::kSlots;
+  constexpr bool STAGE_S = Shape<R>::kStageS && MIE && CB;
+  auto take = [&q](size_t bytes) { unsigned char* r = q; q += (bytes + 15) & ~size_t(15); return r; };
+  Lds<R> L;
+  L.F = MIE ? (R*)take(sizeof(R) * 4 * (p.sca.Nt + 1)) : nullptr;
+  L.rtrec = (R*)take(sizeof(R) * kRtRec * SLOTS);
+  L.cbrec = CB ? (R*)take(sizeof(R) * kCbRec * SLOTS) : nullptr;
+  L.cbdx = CB ? (double*)take(sizeof(double) * kCbDx * SLOTS) : nullptr;
+  L.esc = (R*)take(sizeof(R) * SLOTS * kWaves * 2);
+  L.wcnt = (int*)take(sizeof(int) * 32);
+  L.C = MIE ? (R*)take(sizeof(R) * (p.sca.rp + 1)) : nullptr;
+  L.S = STAGE_S ? (R*)take(sizeof(R) * 4 * (p.sca.Nt + 1)) : nullptr;
+  return L;
+}
+
+// Copy the solve's Mie tables into LDS (the caller synchronises afterwards).
+template <typename R, bool MIE, bool CB>
+MCRT_DEV void stage_tables(const CoopParams<R>& p, const Lds<R>& L, int tid) {
+  if constexpr (MIE) {
+    for (int i = tid; i < 4 * (p.sca.Nt + 1); i += kThreads) L.F[i] = p.sca.F[i];
+    for (int i = tid; i <= p.sca.rp; i += kThreads) L.C[i] = p.sca.csrn[i];
+  }
+  if constexpr (Shape<R>::kStageS && MIE && CB) {
+    for (int i = tid; i < 4 * (p.sca.Nt + 1); i += kThreads) L.S[i] = p.sca.S[i];
+  }
+}
+
+// Rays [rb, re) of the solve p through one workgroup, with the tables staged
+// in L.  Bins are flushed after every iteration into the slab: a Kahan slab
+// private to the workgroup (sums in slab[0, nout), compensations in
+// slab[nout, 2·nout)), or with ATOMIC a plain FP64 slab shared with other
+// chunks (atomic adds).  The energy budget goes to slab[0..4] at the end and
+// the work counters to p.counters.  slots: this workgroup's SLOTS ray-state
+// records (empty again on return).
+template <typename R, int GEO, bool MIE, bool CB, bool ATOMIC>
+MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int64_t re,
+                       double* slab, Slot<R>* slots) {
   constexpr int SLOTS = Shape<R>::kSlots;
   constexpr int SW = SLOTS / 64;  // waves owning slots
   constexpr int CH = Shape<R>::kChunk;
   constexpr bool STAGE_S = Shape<R>::kStageS && MIE && CB;
-  const CoopParams<R>& p = *pp;
-  extern __shared__ __align__(16) unsigned char lds_raw[];
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const bool owner = tid < SLOTS;
-
-  // ---------------------------------------------------------- LDS layout
-  // (every block is a multiple of 16 bytes; must match lds_plan)
-  unsigned char* q = lds_raw;
-  auto take = [&q](size_t bytes) { unsigned char* r = q; q += (bytes + 15) & ~size_t(15); return r; };
-  // F first: its LDS address is then the lookup offset itself
-  R* ldsF = MIE ? (R*)take(sizeof(R) * 4 * (p.sca.Nt + 1)) : nullptr;
-  R* rtrec = (R*)take(sizeof(R) * kRtRec * SLOTS);
-  R* cbrec = CB ? (R*)take(sizeof(R) * kCbRec * SLOTS) : nullptr;
-  double* cbdx = CB ? (double*)take(sizeof(double) * kCbDx * SLOTS) : nullptr;
-  R* esc = (R*)take(sizeof(R) * SLOTS * kWaves * 2);
-  int* wcnt = (int*)take(sizeof(int) * 32);
-  R* ldsC = MIE ? (R*)take(sizeof(R) * (p.sca.rp + 1)) : nullptr;
-  R* ldsS = STAGE_S ? (R*)take(sizeof(R) * 4 * (p.sca.Nt + 1)) : nullptr;
+  R* const ldsF = L.F;
+  R* const rtrec = L.rtrec;
+  R* const cbrec = L.cbrec;
+  double* const cbdx = L.cbdx;
+  R* const esc = L.esc;
+  int* const wcnt = L.wcnt;
 
   ScaView<R> sca = p.sca;  // transport view (F/csrn in LDS)
   if constexpr (MIE) {
-    for (int i = tid; i < 4 * (p.sca.Nt + 1); i += kThreads) ldsF[i] = p.sca.F[i];
-    for (int i = tid; i <= p.sca.rp; i += kThreads) ldsC[i] = p.sca.csrn[i];
-    sca.F = ldsF;
-    sca.csrn = ldsC;
+    sca.F = L.F;
+    sca.csrn = L.C;
   }
-  if constexpr (STAGE_S) {
-    for (int i = tid; i < 4 * (p.sca.Nt + 1); i += kThreads) ldsS[i] = p.sca.S[i];
-  }
-  const R* Sbins = STAGE_S ? (const R*)ldsS : p.sca.S;
+  const R* Sbins = STAGE_S ? (const R*)L.S : p.sca.S;
   const Geo<R> geo{&p};
-
-  // ------------------------------------------------------ ray range of WG
-  const int64_t G = gridDim.x;
-  const int64_t rb = p.N * (int64_t)blockIdx.x / G;
-  const int64_t re = p.N * ((int64_t)blockIdx.x + 1) / G;
-  __syncthreads();
 
   unsigned long long cnt[kCntN] = {};
   double Aacc[5] = {0, 0, 0, 0, 0};
-  double* slab = p.slabs + (size_t)blockIdx.x * 2 * p.nout;
   double* slabc = slab + p.nout;
   const bool can_dt = (p.geo_kind == 1) || (p.nl == 1 && !p.isinf[0]);
 
   // Per-slot ray state lives in HBM between the transport and update phases
   // so that the bin loops run with a small register footprint.
-  Slot<R>* const myslot = (Slot<R>*)p.slot_mem + (size_t)blockIdx.x * SLOTS + (owner ? tid : 0);
+  Slot<R>* const myslot = slots + (owner ? tid : 0);
   int live = 0;           // this slot holds a ray
   int64_t next_ray = rb;  // identical in every thread
   int e_me = 0;
@@ -879,25 +916,25 @@ __global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __r
       // full register chunks, then the remainder in passes of 4, 2 and 1
       int c0 = 0;
       for (; c0 + CH <= per; c0 += CH)
-        rt_pass<CH, R, GEO, MIE>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow,
+        rt_pass<CH, R, GEO, MIE, ATOMIC>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow,
                                  nb, c0, flag, slot2);
       if (CH > 4 && c0 + 4 <= per) {
-        rt_pass<4, R, GEO, MIE>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
+        rt_pass<4, R, GEO, MIE, ATOMIC>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
                                 c0, flag, slot2);
         c0 += 4;
       }
       if (c0 + 2 <= per) {
-        rt_pass<2, R, GEO, MIE>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
+        rt_pass<2, R, GEO, MIE, ATOMIC>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
                                 c0, flag, slot2);
         c0 += 2;
       }
       if (c0 < per)
-        rt_pass<1, R, GEO, MIE>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
+        rt_pass<1, R, GEO, MIE, ATOMIC>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
                                 c0, flag, slot2);
     }
 
     // ------------------------------------------ CB bins × events (cpu.jl:38-160)
-    if (CB && p.compute_cb) cb_pass<R, GEO, MIE>(p, ldsF, Sbins, cbrec, cbdx, slab, slabc, E, tid);
+    if (CB && p.compute_cb) cb_pass<R, GEO, MIE, ATOMIC>(p, ldsF, Sbins, cbrec, cbdx, slab, slabc, E, tid);
     __syncthreads();
 
     // ---------------------- weight update and termination (cpu.jl:263-272)
@@ -943,12 +980,87 @@ __global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __r
     if (tid < 5) {
       double sum = 0.0, cmp = 0.0;
       for (int i = 0; i < SLOTS; ++i) kahan(&sum, &cmp, buf[tid * SLOTS + i]);
-      slab[tid] = sum;
-      slabc[tid] = cmp;
+      if (ATOMIC) {
+        unsafeAtomicAdd(slab + tid, sum - cmp);
+      } else {
+        slab[tid] = sum;
+        slabc[tid] = cmp;
+      }
     }
   }
   for (int c = 0; c < kCntN; ++c)
     if (cnt[c]) atomicAdd(p.counters + c, cnt[c]);
+}
+
+// One solve: workgroup g transports rays [N·g/G, N·(g+1)/G) into slab g.
+template <typename R, int GEO, bool MIE, bool CB>
+__global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __restrict__ pp) {
+  const CoopParams<R>& p = *pp;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const Lds<R> L = lds_layout<R, MIE, CB>(p, lds_raw);
+  stage_tables<R, MIE, CB>(p, L, threadIdx.x);
+  const int64_t G = gridDim.x;
+  const int64_t rb = p.N * (int64_t)blockIdx.x / G;
+  const int64_t re = p.N * ((int64_t)blockIdx.x + 1) / G;
+  __syncthreads();
+  run_rays<R, GEO, MIE, CB, false>(p, L, rb, re, p.slabs + (size_t)blockIdx.x * 2 * p.nout,
+                            (Slot<R>*)p.slot_mem + (size_t)blockIdx.x * Shape<R>::kSlots);
+}
+
+template <typename R, int GEO, bool MIE, bool CB>
+MCRT_DEV void batch_chunk(const CoopParams<R>& p, bool fresh, const BatchChunk& ch,
+                          unsigned char* lds, Slot<R>* slots) {
+  const Lds<R> L = lds_layout<R, MIE, CB>(p, lds);
+  if (fresh) {
+    stage_tables<R, MIE, CB>(p, L, threadIdx.x);
+    __syncthreads();
+  }
+  run_rays<R, GEO, MIE, CB, true>(p, L, ch.rb, ch.re, ch.slab, slots);
+}
+
+// Many solves in one persistent launch: each workgroup takes the next chunk
+// (a ray range of one solve) from the queue until it is empty, restaging the
+// tables when the solve changes.  A chunk's rays do not share iterations with
+// any other chunk, so its per-iteration sums are the same whichever workgroup
+// runs it and whatever else is in the launch; they are added into one of the
+// solve's batch slabs with FP64 atomics.
+template <typename R>
+__global__ __launch_bounds__(kThreads) void batch_kernel(const BatchChunk* __restrict__ chunks,
+                                                         int nchunks, unsigned int* __restrict__ next,
+                                                         void* slot_mem) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ int chunk_id;
+  Slot<R>* const slots = (Slot<R>*)slot_mem + (size_t)blockIdx.x * Shape<R>::kSlots;
+  const void* cur = nullptr;
+  for (;;) {
+    if (threadIdx.x == 0) chunk_id = (int)atomicAdd(next, 1u);
+    __syncthreads();
+    const int c = chunk_id;
+    __syncthreads();
+    if (c >= nchunks) break;
+    const BatchChunk ch = chunks[c];
+    const CoopParams<R>& p = *(const CoopParams<R>*)ch.params;
+    const bool fresh = ch.params != cur;
+    cur = ch.params;
+    const bool mie = p.sca.kind != 0, cb = p.compute_cb != 0;
+    if (p.geo_kind == 0) {
+      if (mie) {
+        if (cb) batch_chunk<R, 0, true, true>(p, fresh, ch, lds_raw, slots);
+        else batch_chunk<R, 0, true, false>(p, fresh, ch, lds_raw, slots);
+      } else {
+        if (cb) batch_chunk<R, 0, false, true>(p, fresh, ch, lds_raw, slots);
+        else batch_chunk<R, 0, false, false>(p, fresh, ch, lds_raw, slots);
+      }
+    } else {
+      if (mie) {
+        if (cb) batch_chunk<R, 1, true, true>(p, fresh, ch, lds_raw, slots);
+        else batch_chunk<R, 1, true, false>(p, fresh, ch, lds_raw, slots);
+      } else {
+        if (cb) batch_chunk<R, 1, false, true>(p, fresh, ch, lds_raw, slots);
+        else batch_chunk<R, 1, false, false>(p, fresh, ch, lds_raw, slots);
+      }
+    }
+  }
 }
 
 // out[i] = Σ_g (sum_g[i] - comp_g[i]) in workgroup order (Kahan).
@@ -990,6 +1102,40 @@ __global__ void slab_sigma(const double* __restrict__ slabs, int nslab, int64_t 
     v += d * d / (double)ng;
   }
   sig[i] = sqrt(v / (double)(nslab - 1) / (double)N);
+}
+
+// Zero n 32-bit words (a kernel rather than a memset so that a captured graph
+// holds no memset node).
+__global__ void zero_words(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// Outputs of one batched solve from its batch slabs (slab b holds the raw sums
+// of the chunks k ≡ b mod nbatch, brays[b] rays): out[i] = Σ_b S_b[i] in batch
+// order, sig[i] the batch-means standard error of out[i]/N (NaN for one batch).
+__global__ void batch_reduce(const double* __restrict__ slabs, int nbatch, int64_t nout, int64_t N,
+                             const int64_t* __restrict__ brays, double* __restrict__ out,
+                             double* __restrict__ sig) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nout) return;
+  double T = 0.0, c = 0.0;
+  for (int b = 0; b < nbatch; ++b) kahan(&T, &c, slabs[(size_t)b * nout + i]);
+  T -= c;
+  out[i] = T;
+  if (nbatch < 2) {
+    sig[i] = NAN;
+    return;
+  }
+  const double mean = T / (double)N;
+  double v = 0.0;
+  for (int b = 0; b < nbatch; ++b) {
+    const double nb = (double)brays[b];
+    const double d = slabs[(size_t)b * nout + i] - nb * mean;
+    v += d * d / nb;
+  }
+  sig[i] = sqrt(v / (double)(nbatch - 1) / (double)N);
 }
 
 // ------------------------------------------------------------------ host
@@ -1047,15 +1193,121 @@ void h_E_from_I(const double I[4], double E[4]) {
   E[0] = A1; E[1] = A1 * 0.0; E[2] = A2 * std::cos(ph); E[3] = A2 * std::sin(ph);
 }
 
-// Upload a scatterer's tables in T, F and S padded with one zero node.
+}  // namespace
+
+// ------------------------------------------------------------- table cache
+struct TableCache {
+  struct Entry {
+    void* F = nullptr;
+    void* S = nullptr;
+    void* C = nullptr;
+    size_t bytes = 0;
+    int refs = 0;
+    uint64_t last_use = 0;
+  };
+  std::mutex mu;
+  std::map<std::pair<uint64_t, int>, Entry*> map;  // (content hash, element size)
+  size_t budget = 0, bytes = 0;
+  uint64_t clock = 0, hits = 0, misses = 0;
+
+  ~TableCache() {
+    for (auto& kv : map) {
+      (void)hipFree(kv.second->F);
+      (void)hipFree(kv.second->S);
+      (void)hipFree(kv.second->C);
+      delete kv.second;
+    }
+  }
+  // Free unreferenced entries, least recently used first, down to the budget.
+  void trim() {
+    while (bytes > budget) {
+      auto victim = map.end();
+      for (auto it = map.begin(); it != map.end(); ++it)
+        if (it->second->refs == 0 && (victim == map.end() || it->second->last_use < victim->second->last_use))
+          victim = it;
+      if (victim == map.end()) return;
+      Entry* e = victim->second;
+      (void)hipFree(e->F);
+      (void)hipFree(e->S);
+      (void)hipFree(e->C);
+      bytes -= e->bytes;
+      delete e;
+      map.erase(victim);
+    }
+  }
+};
+
+TableCache* table_cache_create(size_t budget_bytes) {
+  auto* c = new TableCache();
+  c->budget = budget_bytes;
+  return c;
+}
+void table_cache_destroy(TableCache* c) { delete c; }
+void table_cache_release(TableCache* c, const std::vector<const void*>& refs) {
+  if (!c) return;
+  std::lock_guard<std::mutex> g(c->mu);
+  for (const void* r : refs) {
+    auto* e = (TableCache::Entry*)r;
+    if (e->refs > 0) --e->refs;
+  }
+  c->trim();
+}
+void table_cache_stats(TableCache* c, size_t* bytes, uint64_t* hits, uint64_t* misses) {
+  std::lock_guard<std::mutex> g(c->mu);
+  if (bytes) *bytes = c->bytes;
+  if (hits) *hits = c->hits;
+  if (misses) *misses = c->misses;
+}
+
+namespace {
+
+// FNV-1a over the scatterer's sizes and tables: the cache key.
+uint64_t table_hash(const mcrt_scatterer* s) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+  };
+  mix(&s->n_theta, sizeof s->n_theta);
+  mix(&s->random_points, sizeof s->random_points);
+  const size_t N = (size_t)s->n_theta;
+  mix(s->F11, 8 * N); mix(s->F12, 8 * N); mix(s->F33, 8 * N); mix(s->F34, 8 * N);
+  mix(s->S11, 16 * N); mix(s->S22, 16 * N);
+  mix(s->csrn, 8 * ((size_t)s->random_points + 1));
+  return h;
+}
+
+// Upload a scatterer's tables in R, F and S padded with one zero node: into
+// memory of the solve (alloc), or shared through the cache (a reference is
+// appended to refs).
 template <typename R>
-void fill_sca(ScaView<R>& v, const mcrt_scatterer* s, const DevAlloc& alloc, int* rc, std::string* err) {
+void fill_sca(ScaView<R>& v, const mcrt_scatterer* s, const DevAlloc& alloc, int* rc, std::string* err,
+              TableCache* cache = nullptr, std::vector<const void*>* refs = nullptr) {
   v.kind = s->kind == MCRT_SCATTERER_MIE ? 1 : 0;
   v.Nt = s->n_theta;
   v.rp = s->random_points;
   v.scale = 0;
   v.csrn = v.F = v.S = nullptr;
   if (!v.kind) return;
+  std::unique_lock<std::mutex> lk;
+  std::pair<uint64_t, int> key;
+  if (cache) {
+    lk = std::unique_lock<std::mutex>(cache->mu);
+    key = {table_hash(s), (int)sizeof(R)};
+    auto it = cache->map.find(key);
+    if (it != cache->map.end()) {
+      TableCache::Entry* e = it->second;
+      ++e->refs;
+      e->last_use = ++cache->clock;
+      ++cache->hits;
+      refs->push_back(e);
+      v.F = (const R*)e->F;
+      v.S = (const R*)e->S;
+      v.csrn = (const R*)e->C;
+      return;
+    }
+    ++cache->misses;
+  }
   const int N = s->n_theta;
   std::vector<R> F(4 * (size_t)(N + 1), R(0)), S(4 * (size_t)(N + 1), R(0)),
       C((size_t)s->random_points + 1);
@@ -1068,7 +1320,13 @@ void fill_sca(ScaView<R>& v, const mcrt_scatterer* s, const DevAlloc& alloc, int
   for (int i = 0; i <= s->random_points; ++i) C[i] = (R)s->csrn[i];
   auto up = [&](const std::vector<R>& h) -> const R* {
     hipError_t e;
-    R* d = (R*)alloc(sizeof(R) * h.size(), &e);
+    R* d = nullptr;
+    if (cache) {
+      e = hipMalloc((void**)&d, sizeof(R) * h.size());
+      if (e != hipSuccess) d = nullptr;
+    } else {
+      d = (R*)alloc(sizeof(R) * h.size(), &e);
+    }
     if (!d) { *rc = MCRT_ERR_NOMEM; *err = "coop: table allocation failed"; return nullptr; }
     if (hipMemcpy(d, h.data(), sizeof(R) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
       *rc = MCRT_ERR_HIP; *err = "coop: table upload failed"; return nullptr;
@@ -1078,6 +1336,24 @@ void fill_sca(ScaView<R>& v, const mcrt_scatterer* s, const DevAlloc& alloc, int
   v.F = up(F);
   if (v.F) v.S = up(S);
   if (v.S) v.csrn = up(C);
+  if (cache) {
+    if (!v.csrn) {
+      (void)hipFree((void*)v.F);
+      (void)hipFree((void*)v.S);
+      return;
+    }
+    auto* e = new TableCache::Entry();
+    e->F = (void*)v.F;
+    e->S = (void*)v.S;
+    e->C = (void*)v.csrn;
+    e->bytes = sizeof(R) * (F.size() + S.size() + C.size());
+    e->refs = 1;
+    e->last_use = ++cache->clock;
+    cache->map[key] = e;
+    cache->bytes += e->bytes;
+    refs->push_back(e);
+    cache->trim();
+  }
 }
 
 // LDS bytes of one workgroup (the layout of coop_kernel).
@@ -1092,12 +1368,14 @@ template <typename R> size_t lds_plan(bool mie, bool cb, int nt, int rp) {
   return b;
 }
 
+// Fill the parameter block of one solve and upload its tables and bin
+// constants (everything but the launch resources: slabs and slot state).
 template <typename R>
-int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* cfg,
+int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_config* cfg,
                 const mcrt_geometry* g, const double I0[4], uint64_t seed, uint64_t ray0,
-                uint32_t instance, const DevAlloc& alloc,
-                unsigned long long* dcnt, std::string* err) {
-  CoopParams<R> P;
+                uint32_t instance, const DevAlloc& alloc, unsigned long long* dcnt,
+                std::string* err, TableCache* cache = nullptr,
+                std::vector<const void*>* refs = nullptr) {
   std::memset(&P, 0, sizeof P);
   // one scatterer for all layers
   const mcrt_scatterer* s0 = g->kind == MCRT_GEOMETRY_PLANE ? g->layers[0].scatterer : g->scatterer;
@@ -1130,9 +1408,9 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
   }
   P.kl1 = gd.kl1;
   int rc = MCRT_OK;
-  fill_sca<R>(P.sca, s0, alloc, &rc, err);
+  fill_sca<R>(P.sca, s0, alloc, &rc, err, cache, refs);
   if (rc) return rc;
-  fill_sca<double>(P.sca64, s0, alloc, &rc, err);
+  fill_sca<double>(P.sca64, s0, alloc, &rc, err, cache, refs);
   if (rc) return rc;
   {
     const double sc = mie ? (s0->n_theta - 1) / 3.141592653589793 : 0.0;
@@ -1229,6 +1507,30 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
   P.ncb = 4 * (int64_t)cfg->n_theta_b * cfg->n_phi_b;
   P.nout = 5 + P.nrt + P.ncb;
   P.counters = dcnt;
+  *lds_out = lds;
+  return MCRT_OK;
+}
+
+template <typename R>
+int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* cfg,
+                const mcrt_geometry* g, const double I0[4], uint64_t seed, uint64_t ray0,
+                uint32_t instance, const DevAlloc& alloc,
+                unsigned long long* dcnt, std::string* err) {
+  CoopParams<R> P;
+  size_t lds = 0;
+  int rc = init_params<R>(P, &lds, lds_max, cfg, g, I0, seed, ray0, instance, alloc, dcnt, err);
+  if (rc) return rc;
+  const bool mie = P.sca.kind != 0;
+  const bool cb = P.compute_cb != 0;
+  auto up = [&](const void* h, size_t bytes) -> void* {
+    hipError_t e;
+    void* d = alloc(bytes, &e);
+    if (!d) { rc = MCRT_ERR_NOMEM; *err = "coop: allocation failed"; return nullptr; }
+    if (hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      rc = MCRT_ERR_HIP; *err = "coop: upload failed"; return nullptr;
+    }
+    return d;
+  };
   // launch shape: LDS bounds the workgroups per CU; at most one per SLOTS rays
   cp.lds_bytes = lds;
   const int wg_per_cu = std::max<int>(1, (int)(lds_max / std::max<size_t>(lds, 1)));
@@ -1300,6 +1602,143 @@ hipError_t coop_prepare(CoopPlan& cp, hipStream_t s) {
 
 hipError_t coop_launch(CoopPlan& cp, hipStream_t s) {
   return cp.mixed ? launch_r<float>(cp, s) : launch_r<double>(cp, s);
+}
+
+// ------------------------------------------------------------- batched solves
+constexpr int kBatchSlabs = 32;
+
+int64_t default_chunk_rays(int64_t N) {
+  int64_t c = (N + 63) / 64;
+  c = std::max<int64_t>(512, std::min<int64_t>(8192, c));
+  if (N >= 2 && c >= N) c = (N + 1) / 2;  // at least two chunks: a standard error
+  return std::max<int64_t>(1, c);
+}
+
+namespace {
+template <typename R>
+int solve_create_impl(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, const mcrt_geometry* g,
+                      const double I0[4], uint64_t seed, uint64_t ray0, uint32_t instance,
+                      int64_t chunk_rays, const DevAlloc& alloc, std::string* err,
+                      TableCache* cache) {
+  hipError_t e;
+  sd.counters = (unsigned long long*)alloc(sizeof(unsigned long long) * kCntN, &e);
+  if (!sd.counters) { *err = "batch: counter allocation failed"; return MCRT_ERR_NOMEM; }
+  CoopParams<R> P;
+  size_t lds = 0;
+  int rc = init_params<R>(P, &lds, lds_max, cfg, g, I0, seed, ray0, instance, alloc, sd.counters, err,
+                          cache, &sd.table_refs);
+  if (rc) return rc;
+  sd.mixed = sizeof(R) == 4;
+  sd.geo_kind = P.geo_kind;
+  sd.mie = P.sca.kind != 0;
+  sd.cb = P.compute_cb != 0;
+  sd.N = P.N;
+  sd.nout = P.nout;
+  sd.nrt = P.nrt;
+  sd.ncb = P.ncb;
+  sd.lds_bytes = lds;
+  sd.chunk_rays = chunk_rays > 0 ? chunk_rays : default_chunk_rays(P.N);
+  sd.nchunks = (P.N + sd.chunk_rays - 1) / sd.chunk_rays;
+  sd.nbatch = (int)std::min<int64_t>(kBatchSlabs, sd.nchunks);
+  std::vector<int64_t> br(sd.nbatch, 0);
+  for (int64_t k = 0; k < sd.nchunks; ++k)
+    br[k % sd.nbatch] += std::min(P.N, (k + 1) * sd.chunk_rays) - k * sd.chunk_rays;
+  sd.slabs = (double*)alloc(sizeof(double) * (size_t)sd.nbatch * (size_t)sd.nout, &e);
+  sd.batch_rays = (int64_t*)alloc(sizeof(int64_t) * br.size(), &e);
+  sd.out = (double*)alloc(sizeof(double) * (size_t)sd.nout, &e);
+  sd.sig = (double*)alloc(sizeof(double) * (size_t)sd.nout, &e);
+  void* pd = alloc(sizeof P, &e);
+  if (!sd.slabs || !sd.batch_rays || !sd.out || !sd.sig || !pd) {
+    *err = "batch: solve allocation failed";
+    return MCRT_ERR_NOMEM;
+  }
+  if (hipMemcpy(sd.batch_rays, br.data(), sizeof(int64_t) * br.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(pd, &P, sizeof P, hipMemcpyHostToDevice) != hipSuccess) {
+    *err = "batch: upload failed";
+    return MCRT_ERR_HIP;
+  }
+  sd.params_dev = pd;
+  // relative work of one chunk: rays × bin evaluations per event × events per ray
+  // (a rough optical-depth model; only the order of chunks depends on it)
+  const double tau = g->kind == MCRT_GEOMETRY_PLANE ? g->layers[0].tau : g->tau;
+  const double ev = 1.0 + 2.0 * tau + 0.5 * tau * tau;
+  sd.cost = (double)sd.chunk_rays * (double)(P.nrt / 4 + 3 * (P.ncb / 4) + 1) * ev;
+  return MCRT_OK;
+}
+}  // namespace
+
+int solve_create(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, const mcrt_geometry* geom,
+                 const double I0[4], uint64_t seed, uint64_t ray0, uint32_t instance, bool mixed,
+                 int64_t chunk_rays, const DevAlloc& alloc, std::string* err, TableCache* cache) {
+  if (lds_max == 0) lds_max = 160 * 1024;
+  lds_max -= 16;  // the batch kernel's static LDS (the chunk index)
+  return mixed ? solve_create_impl<float>(sd, lds_max, cfg, geom, I0, seed, ray0, instance,
+                                          chunk_rays, alloc, err, cache)
+               : solve_create_impl<double>(sd, lds_max, cfg, geom, I0, seed, ray0, instance,
+                                           chunk_rays, alloc, err, cache);
+}
+
+size_t solve_bytes_estimate(const mcrt_config* cfg, bool mixed, int64_t chunk_rays) {
+  const int64_t N = cfg->number_of_rays;
+  const int64_t cr = chunk_rays > 0 ? chunk_rays : default_chunk_rays(N);
+  const int64_t nb = std::min<int64_t>(kBatchSlabs, (N + cr - 1) / cr);
+  const int64_t nbins = cfg->n_theta * cfg->n_phi + (cfg->compute_cb ? cfg->n_theta_b * cfg->n_phi_b : 0);
+  const int64_t nout = 5 + 4 * nbins;
+  const size_t r = mixed ? 4 : 8;
+  return (size_t)(nb + 2) * (size_t)nout * 8 + (size_t)nbins * kBinStride * r +
+         (size_t)nbins * 32 + sizeof(CoopParams<double>) + 4096;
+}
+
+void solve_chunk(const SolveDev& sd, int64_t k, BatchChunk* out) {
+  out->params = sd.params_dev;
+  out->rb = k * sd.chunk_rays;
+  out->re = std::min(sd.N, (k + 1) * sd.chunk_rays);
+  out->slab = sd.slabs + (size_t)(k % sd.nbatch) * (size_t)sd.nout;
+}
+
+static hipError_t zero(void* p, size_t bytes, hipStream_t s) {
+  const int64_t n = (int64_t)(bytes / 4);
+  const int bl = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 255) / 256));
+  hipLaunchKernelGGL(zero_words, dim3(bl), dim3(256), 0, s, (uint32_t*)p, n);
+  return hipGetLastError();
+}
+
+hipError_t solve_prepare(const SolveDev& sd, hipStream_t s) {
+  hipError_t e = zero(sd.slabs, sizeof(double) * (size_t)sd.nbatch * (size_t)sd.nout, s);
+  if (e == hipSuccess) e = zero(sd.counters, sizeof(unsigned long long) * kCntN, s);
+  return e;
+}
+
+hipError_t solve_finish(const SolveDev& sd, hipStream_t s) {
+  const int th = 256;
+  const int bl = (int)((sd.nout + th - 1) / th);
+  hipLaunchKernelGGL(batch_reduce, dim3(bl), dim3(th), 0, s, (const double*)sd.slabs, sd.nbatch,
+                     sd.nout, sd.N, (const int64_t*)sd.batch_rays, sd.out, sd.sig);
+  return hipGetLastError();
+}
+
+size_t batch_slot_bytes(bool mixed, int blocks) {
+  return mixed ? sizeof(Slot<float>) * (size_t)blocks * Shape<float>::kSlots
+               : sizeof(Slot<double>) * (size_t)blocks * Shape<double>::kSlots;
+}
+
+hipError_t batch_set_lds(bool mixed, size_t lds_bytes) {
+  const void* k = mixed ? (const void*)batch_kernel<float> : (const void*)batch_kernel<double>;
+  return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+}
+
+hipError_t batch_launch(bool mixed, const BatchChunk* dchunks, int nchunks, unsigned int* dnext,
+                        void* slot_mem, int blocks, size_t lds_bytes, hipStream_t s) {
+  hipError_t e = zero(dnext, sizeof(unsigned int), s);
+  if (e != hipSuccess) return e;
+  if (mixed)
+    hipLaunchKernelGGL(batch_kernel<float>, dim3(blocks), dim3(kThreads), lds_bytes, s, dchunks,
+                       nchunks, dnext, slot_mem);
+  else
+    hipLaunchKernelGGL(batch_kernel<double>, dim3(blocks), dim3(kThreads), lds_bytes, s, dchunks,
+                       nchunks, dnext, slot_mem);
+  return hipGetLastError();
 }
 
 hipError_t coop_sigma(CoopPlan& cp, hipStream_t s, int64_t N, double* sig) {

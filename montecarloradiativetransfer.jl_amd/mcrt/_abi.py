@@ -125,6 +125,36 @@ class mcrt_stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class mcrt_request(C.Structure):
+    _fields_ = [
+        ("cfg", C.POINTER(mcrt_config)),
+        ("geom", C.POINTER(mcrt_geometry)),
+        ("I0", C.c_double * 4),
+        ("seed", C.c_uint64),
+        ("ray_begin", C.c_uint64),
+        ("instance", C.c_uint32),
+        ("precision", C.c_int32),
+        ("chunk_rays", C.c_int64),
+    ]
+
+
+class mcrt_server_stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "cancelled", "failed",
+                                          "launches", "chunks", "rays", "queued", "running")] + [
+        ("device_bytes_in_use", C.c_int64),
+        ("device_budget", C.c_int64),
+        ("table_cache_bytes", C.c_int64),
+        ("table_hits", C.c_uint64),
+        ("table_misses", C.c_uint64),
+        ("busy_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+TICKET_STATES = {0: "queued", 1: "running", 2: "done", 3: "cancelled", 4: "failed"}
+
 # Every function declared in include/mcrt.h: name -> (restype, argtypes).
 _vp = C.c_void_p
 SIGNATURES = {
@@ -150,6 +180,23 @@ SIGNATURES = {
                                          C.c_uint64, C.c_int32, C.c_uint64, C.c_uint32,
                                          C.POINTER(_vp)]),
     "mcrt_plan_read_sigma": (C.c_int, [_vp, _dp, _dp, _dp, C.POINTER(C.c_int32)]),
+    "mcrt_batch_create": (C.c_int, [_vp, C.c_int32, C.POINTER(mcrt_request), C.POINTER(_vp)]),
+    "mcrt_batch_launch": (C.c_int, [_vp]),
+    "mcrt_batch_wait": (C.c_int, [_vp]),
+    "mcrt_batch_read": (C.c_int, [_vp, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp,
+                                  C.POINTER(mcrt_stats)]),
+    "mcrt_batch_kernel_ms": (C.c_int, [_vp, _dp]),
+    "mcrt_batch_info": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "mcrt_batch_destroy": (None, [_vp]),
+    "mcrt_server_create": (C.c_int, [_vp, C.c_int64, C.c_int64, C.POINTER(_vp)]),
+    "mcrt_submit": (C.c_int, [_vp, C.POINTER(mcrt_request), C.POINTER(C.c_uint64)]),
+    "mcrt_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(C.c_int32)]),
+    "mcrt_wait": (C.c_int, [_vp, C.c_uint64, C.c_double, C.POINTER(C.c_int32)]),
+    "mcrt_result": (C.c_int, [_vp, C.c_uint64, _dp, _dp, _dp, _dp, _dp, _dp,
+                              C.POINTER(mcrt_stats), _dp]),
+    "mcrt_cancel": (C.c_int, [_vp, C.c_uint64]),
+    "mcrt_server_stats_get": (C.c_int, [_vp, C.POINTER(mcrt_server_stats)]),
+    "mcrt_server_destroy": (None, [_vp]),
     "mcrt_plan_destroy": (None, [_vp]),
 }
 

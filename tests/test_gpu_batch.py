@@ -1,0 +1,110 @@
+"""Batched solves (one persistent launch per precision) against single solves."""
+import numpy as np
+import pytest
+
+import mcrt
+from mcrt import workloads as W
+from mcrt.batch import Batch, Request, random_instances
+from statcheck import compare, flat
+
+pytestmark = pytest.mark.gpu
+
+I0 = (1.0, -1.0, 0.0, 0.0)
+
+
+def _mix():
+    a = W.plane_rayleigh_tau1(4096, compute_cb=True)
+    b = W.sphere_mie_cb(4096, Ntheta_table=1801)
+    c = W.sphere_mie_cb(2048, Ntheta_table=1801)
+    d = W.plane_rayleigh_tau1(4096)
+    return [Request(a[0], a[1], I0, seed=a[2], precision="mixed"),
+            Request(b[0], b[1], I0, seed=b[2], precision="mixed"),
+            Request(c[0], c[1], I0, seed=c[2] + 1, precision="fp64"),
+            Request(d[0], d[1], I0, seed=d[2], precision="fp64")]
+
+
+def _tol(form, ref):
+    return (1e-5 if form == "mixed" else 1e-11) * float(np.max(np.abs(ref)))
+
+
+def test_batch_matches_single_solves():
+    reqs = _mix()
+    print(f"batch of {len(reqs)}", flush=True)
+    with Batch(reqs) as b:
+        b.launch()
+        b.wait()
+        print(f"batch info {b.info()} {b.kernel_ms():.1f} ms", flush=True)
+        for i, r in enumerate(reqs):
+            st = {}
+            got = flat(b.read(i, stats=st))
+            ref = flat(mcrt.simulate(r.cfg, r.g, r.I0, seed=r.seed, precision=r.precision))
+            err = float(np.max(np.abs(got - ref)))
+            print(f"[batch/{i}/{r.precision}] rays={st['rays']} max|Δ|={err:.3e} "
+                  f"tol={_tol(r.precision, ref):.3e}", flush=True)
+            assert st["rays"] == r.cfg.number_of_rays and st["nonfinite"] == 0
+            assert abs(sum(got[:5]) - 1.0) < (1e-5 if r.precision == "mixed" else 1e-9)
+            assert err <= _tol(r.precision, ref)
+
+
+def test_alone_equals_in_the_mix():
+    reqs = _mix()
+    alone = Batch([reqs[1]])
+    mixed = Batch(reqs)
+    for x in (alone, mixed):
+        x.launch()
+        x.wait()
+    a = flat(alone.read(0))
+    m = flat(mixed.read(1))
+    err = float(np.max(np.abs(a - m)))
+    print(f"[alone-vs-mix] max|Δ|={err:.3e}", flush=True)
+    assert err <= 1e-12 * float(np.max(np.abs(a)))
+    alone.close()
+    mixed.close()
+
+
+def test_graph_replays_agree_with_the_first_launch():
+    reqs = _mix()[:3]
+    with Batch(reqs) as b:
+        outs = []
+        for k in range(3):
+            b.launch()
+            b.wait()
+            outs.append([flat(b.read(i)) for i in range(len(reqs))])
+            # eager work of the program between replays
+            mcrt.simulate(reqs[0].cfg.replace(number_of_rays=256), reqs[0].g, I0, seed=k,
+                          precision="mixed")
+        for k in (1, 2):
+            for i in range(len(reqs)):
+                err = float(np.max(np.abs(outs[k][i] - outs[0][i])))
+                assert err <= 1e-12 * float(np.max(np.abs(outs[0][i]))), (k, i, err)
+
+
+def test_batch_sigma_consistent_with_independent_streams():
+    cfg, g, seed = W.sphere_mie_cb(32768, Ntheta_table=1801)
+    reqs = [Request(cfg, g, I0, seed=seed, precision="mixed"),
+            Request(cfg, g, I0, seed=seed + 1, precision="mixed")]
+    with Batch(reqs) as b:
+        b.launch()
+        b.wait()
+        sa, sb = {}, {}
+        a = flat(b.read(0, sigma=sa))
+        c = flat(b.read(1, sigma=sb))
+    s = np.sqrt(flat((*sa["A"], sa["Irt"], sa["Icb"])) ** 2 + flat((*sb["A"], sb["Irt"], sb["Icb"])) ** 2)
+    r = compare(a, c, s, floor=1e-15, dead_rel=1e-6, dof=31, label="batch-sigma")
+    assert r["ok"], r
+    assert r["chi2dof"] > 0.7, r
+
+
+def test_random_population_runs():
+    reqs = random_instances(8, seed=7, ray_scale=1e-3)
+    with Batch(reqs) as b:
+        b.launch()
+        b.wait()
+        for i, r in enumerate(reqs):
+            st = {}
+            out = flat(b.read(i, stats=st))
+            print(f"[pop/{i}] {r.meta['geometry']} {r.meta['scatterer']} tau={r.meta['tau']:.2f} "
+                  f"rays={st['rays']} events={st['events']}", flush=True)
+            assert st["rays"] == r.cfg.number_of_rays
+            assert np.all(np.isfinite(out))
+            assert abs(sum(out[:5]) - 1.0) < 1e-5
