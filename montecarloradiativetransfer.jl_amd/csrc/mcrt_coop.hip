@@ -1605,7 +1605,10 @@ hipError_t coop_launch(CoopPlan& cp, hipStream_t s) {
 }
 
 // ------------------------------------------------------------- batched solves
-constexpr int kBatchSlabs = 32;
+// Batch slabs per solve: the batch-means σ needs enough batches for bins whose
+// per-ray contributions are sparse (a few large values), where few batches
+// underestimate the variance.
+constexpr int kBatchSlabs = 128;
 
 int64_t default_chunk_rays(int64_t N) {
   int64_t c = (N + 63) / 64;

@@ -133,14 +133,14 @@ BATCH_THETAB_DEG = np.concatenate([np.arange(101) * 0.05, np.arange(6.0, 181.0)]
 
 def random_instances(n: int = 256, *, seed: int = 2024, ray_scale: float = 1.0,
                      Ntheta_table: int = 1801, random_points: int = 1000, precision="mixed",
-                     lam: float = 0.55, nu: float = 0.1):
+                     lam: float = 0.55, nu: float = 0.1, min_rays: int = 1000):
     """The batched-benchmark population, drawn from `seed`:
 
     50/50 plane slab (normal incidence) / sphere; optical thickness or radius
     τ log-uniform in [0.1, 20]; 25% Rayleigh (ω = 1, kl = 1), else Mie spheres
     with size parameter x uniform in [0.5, 20], Re m uniform in [1.3, 1.7] and
     Im m log-uniform in [1e-5, 1e-1] at λ = 0.55 µm, ν = 0.1; rays log-uniform
-    in [1e5, 1e8] times ray_scale (at least 1000); CB on; RT 180 × 1° polar
+    in [1e5, 1e8] times ray_scale (at least min_rays); CB on; RT 180 × 1° polar
     midpoint bins × 48 azimuths; CB θb 0-5° every 0.05° plus 6-180° every 1°,
     8 azimuths.  Returns a list of Request (meta holds the drawn parameters)."""
     rng = np.random.default_rng(seed)
@@ -152,7 +152,7 @@ def random_instances(n: int = 256, *, seed: int = 2024, ray_scale: float = 1.0,
         x = float(rng.uniform(0.5, 20.0))
         mre = float(rng.uniform(1.3, 1.7))
         mim = _loguniform(rng, 1e-5, 1e-1)
-        N = max(1000, int(round(_loguniform(rng, 1e5, 1e8) * ray_scale)))
+        N = max(min_rays, int(round(_loguniform(rng, 1e5, 1e8) * ray_scale)))
         if rayleigh:
             sca = RayleighScatterer(1.0, 1.0)
         else:

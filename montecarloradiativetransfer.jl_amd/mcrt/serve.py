@@ -196,7 +196,7 @@ def poisson_load(srv: Server, pool: list[Request], *, rate: float, duration: flo
 
 
 def main(argv=None) -> int:
-    """python -m mcrt.serve [--rate 200] [--duration 60] [--ray-scale 1e-4] [--check 16]
+    """python -m mcrt.serve [--rate 200] [--duration 60] [--ray-scale 5e-5] [--check 16]
 
     Poisson load drawn from the 256-instance batched population (mixed form),
     then an FP64 re-run of the first `--check` requests on the same streams
@@ -210,21 +210,23 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=60.0)
-    ap.add_argument("--ray-scale", type=float, default=1e-4)
+    ap.add_argument("--ray-scale", type=float, default=5e-5)
     ap.add_argument("--population-seed", type=int, default=2024)
     ap.add_argument("--arrival-seed", type=int, default=5)
     ap.add_argument("--check", type=int, default=16)
     ap.add_argument("--budget-gib", type=float, default=64.0)
+    ap.add_argument("--min-rays", type=int, default=100)
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    pool = random_instances(256, seed=a.population_seed, ray_scale=a.ray_scale)
+    pool = random_instances(256, seed=a.population_seed, ray_scale=a.ray_scale,
+                            min_rays=a.min_rays)
     log(f"pool: {len(pool)} instances, mean rays {np.mean([r.cfg.number_of_rays for r in pool]):.0f}")
     with Server(device_budget=int(a.budget_gib * 2**30)) as srv:
         rep = poisson_load(srv, pool, rate=a.rate, duration=a.duration, seed=a.arrival_seed,
                            keep=a.check, log=log)
     kept = rep.pop("kept")
     # per-request agreement with FP64 on the same stream (batched FP64 re-run)
-    from .stats import compare_results
+    from .stats import batch_count, compare_results
     reqs64 = [Request(r.cfg, r.g, r.I0, seed=r.seed, precision="fp64", chunk_rays=r.chunk_rays)
               for r, _ in kept]
     checks = []
@@ -235,7 +237,8 @@ def main(argv=None) -> int:
             for i, (r, out) in enumerate(kept):
                 sg = {}
                 ref = b.read(i, sigma=sg)
-                checks.append(compare_results(out, ref, sg, same_stream=True))
+                nb = batch_count(r.cfg.number_of_rays, r.chunk_rays)
+                checks.append(compare_results(out, ref, sg, same_stream=True, batches=nb))
     rep["fp64_check"] = {"requests": len(checks),
                          "all_within_3sigma": bool(all(c["ok"] for c in checks)),
                          "worst_exceed3_frac": max((c["exceed3_frac"] for c in checks), default=0.0),

@@ -34,7 +34,9 @@ def compare(a, b, sigma, *, floor=0.0, dead_rel=1e-12, dof=None, label="", verbo
     if dof:
         from scipy.stats import t as student_t
         p = float(2 * student_t.sf(3.0, dof))
-        ez2 = dof / (dof - 2.0)
+        # E[z²] is finite only for dof > 2, and χ²/dof is a usable test only
+        # when its spread is moderate: below 5 dof the exceedance count decides
+        ez2 = dof / (dof - 2.0) if dof >= 5 else math.inf
     else:
         p, ez2 = 0.0027, 1.0
     kmax = n * p + 4 * math.sqrt(max(n * p * (1 - p), 1e-12)) + 3
@@ -58,14 +60,30 @@ def flat(res_tuple):
     return np.concatenate([A, Irt, Icb])
 
 
-def compare_results(a, b, sigma_b, *, same_stream=True, label="", verbose=False):
+def default_chunk_rays(N: int) -> int:
+    """The library's chunking rule for batched and served solves (mcrt_coop.hip,
+    default_chunk_rays): ⌈N/64⌉ clamped to [512, 8192], at least two chunks."""
+    c = max(512, min(8192, -(-N // 64)))
+    if N >= 2 and c >= N:
+        c = (N + 1) // 2
+    return max(1, c)
+
+
+def batch_count(N: int, chunk_rays: int = 0) -> int:
+    """Batches behind the σ of a batched or served solve (at most 128)."""
+    c = chunk_rays if chunk_rays > 0 else default_chunk_rays(N)
+    return min(128, -(-N // c))
+
+
+def compare_results(a, b, sigma_b, *, same_stream=True, batches=32, label="", verbose=False):
     """Agreement of two simulate 7-tuples: `a` against `b` with b's standard
-    errors (a dict with "A", "Irt", "Icb").  With same_stream the two used the
-    same Philox stream and only b's σ counts (√2σ otherwise)."""
+    errors (a dict with "A", "Irt", "Icb") estimated from `batches` batch means.
+    With same_stream the two used the same Philox stream and only b's σ counts
+    (√2σ otherwise)."""
     s = flat((*sigma_b["A"], sigma_b["Irt"], sigma_b["Icb"]))
     if not same_stream:
         s = s * math.sqrt(2.0)
-    r = compare(flat(a), flat(b), s, floor=1e-15, dead_rel=1e-6, dof=31, label=label,
-                verbose=verbose)
+    r = compare(flat(a), flat(b), s, floor=1e-15, dead_rel=1e-6, dof=max(1, batches - 1),
+                label=label, verbose=verbose)
     r["exceed3_frac"] = r["exceed3"] / max(r["n"], 1)
     return r
