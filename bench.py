@@ -6,9 +6,10 @@ A step is one full solve (one launch over all rays of the workload, inputs
 resident in HBM).  The workload is the sphere of Mie scatterers with coherent
 backscattering: optical radius 5, r = 0.5 µm, λ = 0.55 µm, m = 1.5+0.001i,
 ν = 0.1, F/S on 1801 angles, RT 180×48 bins, CB 101×8 bins, 1e7 rays, seed 7.
-`value` is rays/s of the fastest built form; the other forms and the oracle on
-the host cores are timed beside it (on bounded samples, stated in the line).
-Prints one JSON line.
+`value` is rays/s of the fastest built form (mixed); the FP64 forms and the
+oracle on the host cores are timed beside it on bounded samples of the same
+stream (the same rays, so the Monte Carlo error is matched), and a short
+Poisson serving run gives the p50/p99 latency.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -103,6 +104,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=8192,
                     help="rays of the oracle CPU baseline run")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="seconds of Poisson serving load (200 req/s) for the p50/p99 latency; 0: skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -181,8 +184,29 @@ def main() -> int:
                          f"{nth} threads, {dt:.1f} s)"}
         log(f"bench: cpu oracle {ncpu / dt:.4g} rays/s on {nth} threads")
 
+    serving = None
+    if args.serve_seconds > 0:
+        from mcrt.batch import random_instances
+        from mcrt.serve import Server, poisson_load
+
+        scale, rate = 5e-5, 200.0
+        pool = random_instances(256, seed=2024, ray_scale=scale, min_rays=100)
+        with Server() as srv:
+            rep = poisson_load(srv, pool, rate=rate, duration=args.serve_seconds, seed=5)
+        serving = {"workload": "Poisson arrivals from the 256-instance batched population "
+                               "(mixed form, rays scaled by 5e-5, at least 100)",
+                   "rate_per_s": rate, "seconds": args.serve_seconds,
+                   "requests": rep["requests"], "completed": rep["completed"],
+                   "rays_per_s": rep["rays_per_s"], "p50_ms": rep.get("p50_ms"),
+                   "p99_ms": rep.get("p99_ms"), "launches": rep["server"]["launches"],
+                   "table_hits": rep["server"]["table_hits"],
+                   "table_misses": rep["server"]["table_misses"]}
+        log(f"bench: serving {rep['completed']}/{rep['requests']} requests, "
+            f"{rep['rays_per_s']:.4g} rays/s, p50 {rep.get('p50_ms', 0):.1f} ms, "
+            f"p99 {rep.get('p99_ms', 0):.1f} ms")
+
     line = {
-        "metric": "rays/sec on 1 GPU (sphere Mie RT+CB)",
+        "metric": "rays/sec on 1 GPU, mixed vs FP64 plain at matched MC error; p50/p99 latency",
         "value": value,
         "unit": "rays/s",
         "n_gpus": 1,
@@ -208,6 +232,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "events_per_ray": st["events"] / N,
         "compare": compare,
+        "serving": serving,
     }
     print(json.dumps(line))
     return 0

@@ -380,10 +380,11 @@ int create_plain(mcrt_plan* P, const mcrt_config* c, const mcrt_geometry* g, con
 
 }  // namespace
 
-extern "C" int mcrt_plan_create_range(mcrt_ctx* ctx, const mcrt_config* cfg,
-                                      const mcrt_geometry* geom, const double I0[4], uint64_t seed,
-                                      int32_t precision, uint64_t ray_begin, uint32_t instance,
-                                      mcrt_plan** out) {
+extern "C" int mcrt_plan_create_shaped(mcrt_ctx* ctx, const mcrt_config* cfg,
+                                       const mcrt_geometry* geom, const double I0[4], uint64_t seed,
+                                       int32_t precision, uint64_t ray_begin, uint32_t instance,
+                                       const mcrt_launch_shape* shape_in, mcrt_plan** out) {
+  const mcrt_launch_shape shape = shape_in ? *shape_in : mcrt_launch_shape{0, 0};
   if (!ctx) return fail(nullptr, MCRT_ERR_INVALID, "context is NULL");
   if (!out || !I0) return fail(ctx, MCRT_ERR_INVALID, "plan_create: NULL argument");
   *out = nullptr;
@@ -414,7 +415,7 @@ extern "C" int mcrt_plan_create_range(mcrt_ctx* ctx, const mcrt_config* cfg,
     rc = create_plain(P, cfg, geom, I0, seed, ray_begin, instance);
   } else {
     rc = mcrt::coop_create(P->coop, ctx->stream, ctx->cu_count, ctx->lds_per_block, cfg, geom, I0, seed,
-                           ray_begin, instance, precision == MCRT_PRECISION_MIXED,
+                           ray_begin, instance, precision == MCRT_PRECISION_MIXED, shape,
                            [P](size_t n, hipError_t* err) { return P->alloc(n, err); },
                            P->dA, P->dIrt, P->dIcb, P->dcnt, &ctx->err);
     if (rc) g_thread_err = ctx->err;
@@ -432,10 +433,29 @@ extern "C" int mcrt_plan_create_range(mcrt_ctx* ctx, const mcrt_config* cfg,
   return MCRT_OK;
 }
 
+extern "C" int mcrt_plan_create_range(mcrt_ctx* ctx, const mcrt_config* cfg,
+                                      const mcrt_geometry* geom, const double I0[4], uint64_t seed,
+                                      int32_t precision, uint64_t ray_begin, uint32_t instance,
+                                      mcrt_plan** out) {
+  return mcrt_plan_create_shaped(ctx, cfg, geom, I0, seed, precision, ray_begin, instance, nullptr,
+                                 out);
+}
+
 extern "C" int mcrt_plan_create(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geometry* geom,
                                 const double I0[4], uint64_t seed, int32_t precision,
                                 mcrt_plan** out) {
-  return mcrt_plan_create_range(ctx, cfg, geom, I0, seed, precision, 0, 0, out);
+  return mcrt_plan_create_shaped(ctx, cfg, geom, I0, seed, precision, 0, 0, nullptr, out);
+}
+
+extern "C" int mcrt_plan_shape(mcrt_plan* P, mcrt_launch_shape* shape, int32_t* workgroups) {
+  if (!P) return fail(nullptr, MCRT_ERR_INVALID, "plan is NULL");
+  const bool plain = P->precision == MCRT_PRECISION_FP64_PLAIN;
+  if (shape) {
+    shape->slots = plain ? 0 : P->coop.slots;
+    shape->workgroups_per_cu = plain ? 0 : P->coop.wg_per_cu;
+  }
+  if (workgroups) *workgroups = plain ? P->plain_blocks : P->coop.blocks;
+  return MCRT_OK;
 }
 
 extern "C" int mcrt_plan_launch(mcrt_plan* P) {

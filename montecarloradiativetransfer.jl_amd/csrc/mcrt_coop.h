@@ -16,6 +16,8 @@ namespace mcrt {
 struct CoopPlan {
   bool mixed = false;
   int blocks = 0;
+  int slots = 0;      // active ray slots per workgroup
+  int wg_per_cu = 0;  // workgroups launched per CU
   int threads = 0;
   size_t lds_bytes = 0;
   void* params_dev = nullptr;  // CoopParams<R> in device memory
@@ -99,7 +101,7 @@ hipError_t batch_launch(bool mixed, const BatchChunk* dchunks, int nchunks, unsi
 // status and writes a message to *err on failure.
 int coop_create(CoopPlan& cp, hipStream_t stream, int cu_count, size_t lds_max, const mcrt_config* cfg,
                 const mcrt_geometry* geom, const double I0[4], uint64_t seed, uint64_t ray0,
-                uint32_t instance, bool mixed,
+                uint32_t instance, bool mixed, const mcrt_launch_shape& shape,
                 const DevAlloc& alloc, double* dA, double* dIrt, double* dIcb,
                 unsigned long long* dcnt, std::string* err);
 

@@ -65,6 +65,8 @@ template <> struct Shape<double> {
 template <typename R> struct CoopParams {
   int64_t N;
   int64_t maxscat;
+  int slots;      // active ray slots per workgroup (multiple of 64, ≤ Shape<R>::kSlots)
+  int rt_cut;     // 0: no peel-off distance can exceed τ₀ (sphere with 2τ < τ₀)
   R min_int;
   R tau0;
   R tau0k;      // tau0 · ExpK<R>::k
@@ -98,7 +100,7 @@ template <typename R> struct CoopParams {
   uint64_t seed;
   uint64_t ray0;  // stream number of the first ray
   uint32_t instance;
-  void* slot_mem;         // [blocks][SLOTS] Slot<R>: per-slot ray state
+  void* slot_mem;         // per workgroup: SLOTS Slot<R>, then SLOTS SlotAcc
   double* slabs;          // [blocks][2][nout]: Kahan sum, compensation
   int64_t nout;           // 5 + nrt + ncb
   int64_t nrt, ncb;
@@ -267,23 +269,34 @@ template <typename R> MCRT_DEV void frame_matrix(Fr<R> f, R m[8]) {
 // bin with an event direction is rare (a divergent branch that is skipped
 // when no lane of the wave takes it).
 template <typename R> struct Plane { R mu, rx, ry, pn, w, inv; };
+
+// The general case; returns ρ² (q.inv is not set).
 template <typename R>
-MCRT_DEV Plane<R> scat_plane(const R* m, R kx, R ky, R kz, R e1, R e2, R g0, R g1, R g2) {
-  Plane<R> q;
+MCRT_DEV R plane_raw(Plane<R>& q, const R* m, R kx, R ky, R kz, R e1, R e2, R g0, R g1, R g2) {
   q.rx = m[0] * kx + m[3] * ky + m[6] * kz;
   q.ry = m[1] * kx + m[4] * ky;
   q.mu = m[2] * kx + m[5] * ky + m[7] * kz;
   q.pn = m[2] * g0 + m[5] * g1 + m[7] * g2;
   q.w = m[2] * e1 + m[5] * e2;
-  R r2 = q.rx * q.rx + q.ry * q.ry;
-  if (r2 < R(1e-24)) {  // Ehₖ = -ϕ̂, Evₖ = sign(μ)·θ̂
-    const R u = e1 * m[0] + e2 * m[3], v = e1 * m[1] + e2 * m[4];
-    q.rx = R(1);
-    q.ry = R(0);
-    q.pn = -v;
-    q.w = q.mu > R(0) ? -u : (q.mu < R(0) ? u : R(0));
-    r2 = R(1);
-  }
+  return q.rx * q.rx + q.ry * q.ry;
+}
+
+// The polar case (ρ² < 1e-24): Ehₖ = -ϕ̂, Evₖ = sign(μ)·θ̂; returns ρ² := 1.
+template <typename R>
+MCRT_DEV R plane_polar(Plane<R>& q, const R* m, R e1, R e2) {
+  const R u = e1 * m[0] + e2 * m[3], v = e1 * m[1] + e2 * m[4];
+  q.rx = R(1);
+  q.ry = R(0);
+  q.pn = -v;
+  q.w = q.mu > R(0) ? -u : (q.mu < R(0) ? u : R(0));
+  return R(1);
+}
+
+template <typename R>
+MCRT_DEV Plane<R> scat_plane(const R* m, R kx, R ky, R kz, R e1, R e2, R g0, R g1, R g2) {
+  Plane<R> q;
+  R r2 = plane_raw(q, m, kx, ky, kz, e1, e2, g0, g1, g2);
+  if (r2 < R(1e-24)) r2 = plane_polar(q, m, e1, e2);
   q.inv = M<R>::div(R(1), r2);
   return q;
 }
@@ -379,13 +392,29 @@ template <typename R> struct Slot {
   Fr<double> kbs;   // frame of -k₂
 };
 
+// Per-slot running sums of the energy budget and the work counters, kept in
+// HBM beside the slot state (not in registers across the bin loops).
+struct SlotAcc {
+  double A[5];
+  unsigned long long cnt[kCntN];
+};
+
+// The slot state and accumulators of workgroup `block`: SLOTS Slot<R>
+// records followed by SLOTS SlotAcc records.
+template <typename R> MCRT_HD size_t slot_block_bytes() {
+  return (sizeof(Slot<R>) + sizeof(SlotAcc)) * (size_t)Shape<R>::kSlots;
+}
+template <typename R> MCRT_DEV Slot<R>* slot_block(void* base, int block) {
+  return (Slot<R>*)((unsigned char*)base + slot_block_bytes<R>() * (size_t)block);
+}
+
 // One pass of the peel-off (cpu.jl:11-28) over CH register-resident bins per
 // thread and all E events of the iteration.  Thread tid owns the bins
 // lb = tid + kThreads·(c0 + i) of the θ-range [jb, jb + nrow) × Nϕ; the escape
 // sum of each event is wave-reduced and added by lane 0 to esc in chunk order.
 // RT record: m[0..7], I[8..11], J[12..13], K·x[14..16], [17] K·τ of the layer
 // (plane) or K²·(τ² - x·x) (sphere), [18] flags; K = ExpK<R>::k.
-template <int CH, typename R, int GEO, bool MIE, bool ATOMIC>
+template <int CH, typename R, int GEO, bool MIE, bool ATOMIC, bool CUT>
 MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc, double* slab,
                       double* slabc, int E, int tid, int lane, int wave, int jb, int nrow, int nb,
                       int c0, int flag, int slot2) {
@@ -411,11 +440,28 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
     const R* rec = rtrec + kRtRec * e;
     const int fl = (int)rec[18];
     if (!(fl & flag)) continue;
+    // the scattering planes of all CH bins first, with one (rarely taken)
+    // branch for the polar case, so that the lookups and Stokes algebra of the
+    // CH bins form one straight-line block the scheduler can interleave
+    Plane<R> qs[CH];
+    R r2[CH];
+    bool polar = false;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      r2[i] = plane_raw(qs[i], rec, bk[i][0], bk[i][1], bk[i][2], bk[i][3], bk[i][4], bk[i][5],
+                        bk[i][6], bk[i][7]);
+      polar |= r2[i] < R(1e-24);
+    }
+    if (polar) {
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        if (r2[i] < R(1e-24)) r2[i] = plane_polar(qs[i], rec, bk[i][3], bk[i][4]);
+    }
     R part = R(0);
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
-      const Plane<R> q = scat_plane(rec, bk[i][0], bk[i][1], bk[i][2], bk[i][3], bk[i][4],
-                                    bk[i][5], bk[i][6], bk[i][7]);
+      Plane<R>& q = qs[i];
+      q.inv = M<R>::div(R(1), r2[i]);
       R o[4];
       peel_I<R, MIE>(p, F, rec + 8, rec + 12, q, o);
       R t;  // K·distance_to_boundary
@@ -425,7 +471,8 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
         const R kx = bk[i][0] * rec[14] + bk[i][1] * rec[15] + bk[i][2] * rec[16];
         t = sqrt_pos(kx * kx + rec[17]) - kx;
       }
-      const R w = t <= tau0k ? bk[i][8] * ExpK<R>::exp_neg(t) : R(0);
+      // the τ₀ cut-off (cpu.jl:20); CUT = false when no distance can exceed it
+      const R w = (!CUT || t <= tau0k) ? bk[i][8] * ExpK<R>::exp_neg(t) : R(0);
       const R wi = w * q.inv;
       acc[i][0] += o[0] * w;
       acc[i][1] += o[1] * wi;
@@ -447,6 +494,39 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
       if (ATOMIC) unsafeAtomicAdd(slab + o + c, (double)acc[i][c]);
       else kahan(slab + o + c, slabc + o + c, (double)acc[i][c]);
     }
+  }
+}
+
+// All θ-ranges of the peel-off: full register chunks of Shape<R>::kChunk bins,
+// then the remainder in passes of 4, 2 and 1.
+template <typename R, int GEO, bool MIE, bool ATOMIC, bool CUT>
+MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc, double* slab,
+                        double* slabc, int E, int tid, int lane, int wave) {
+  constexpr int CH = Shape<R>::kChunk;
+  for (int rg = 0; rg < p.nranges; ++rg) {
+    const int jb = p.range_b[rg], je = p.range_e[rg];
+    const int nrow = je - jb;
+    const int nb = nrow * p.Np;  // bins of this range
+    const int per = (nb + kThreads - 1) / kThreads;
+    const int flag = p.range_kind[rg] == 1 ? 1 : 2;
+    const int slot2 = p.range_kind[rg] == 1 ? 0 : 1;
+    int c0 = 0;
+    for (; c0 + CH <= per; c0 += CH)
+      rt_pass<CH, R, GEO, MIE, ATOMIC, CUT>(p, F, rtrec, esc, slab, slabc, E, tid, lane, wave, jb,
+                                            nrow, nb, c0, flag, slot2);
+    if (CH > 4 && c0 + 4 <= per) {
+      rt_pass<4, R, GEO, MIE, ATOMIC, CUT>(p, F, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow,
+                                           nb, c0, flag, slot2);
+      c0 += 4;
+    }
+    if (c0 + 2 <= per) {
+      rt_pass<2, R, GEO, MIE, ATOMIC, CUT>(p, F, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow,
+                                           nb, c0, flag, slot2);
+      c0 += 2;
+    }
+    if (c0 < per)
+      rt_pass<1, R, GEO, MIE, ATOMIC, CUT>(p, F, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow,
+                                           nb, c0, flag, slot2);
   }
 }
 
@@ -593,7 +673,7 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
-  const bool owner = tid < SLOTS;
+  const bool owner = tid < p.slots;
   R* const ldsF = L.F;
   R* const rtrec = L.rtrec;
   R* const cbrec = L.cbrec;
@@ -609,8 +689,15 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
   const R* Sbins = STAGE_S ? (const R*)L.S : p.sca.S;
   const Geo<R> geo{&p};
 
-  unsigned long long cnt[kCntN] = {};
-  double Aacc[5] = {0, 0, 0, 0, 0};
+  SlotAcc* const acc = (SlotAcc*)(slots + SLOTS) + (owner ? tid : 0);
+  if (owner) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc->A[c] = 0.0;
+#pragma unroll
+    for (int c = 0; c < kCntN; ++c) acc->cnt[c] = 0ull;
+  }
+  double* const Aacc = acc->A;
+  unsigned long long* const cnt = acc->cnt;
   double* slabc = slab + p.nout;
   const bool can_dt = (p.geo_kind == 1) || (p.nl == 1 && !p.isinf[0]);
 
@@ -906,32 +993,10 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
     const int E = wcnt[31];
 
     // ------------------------------- peel-off: bins × events (cpu.jl:6-31)
-    for (int rg = 0; rg < p.nranges; ++rg) {
-      const int jb = p.range_b[rg], je = p.range_e[rg];
-      const int nrow = je - jb;
-      const int nb = nrow * p.Np;  // bins of this range
-      const int per = (nb + kThreads - 1) / kThreads;
-      const int flag = p.range_kind[rg] == 1 ? 1 : 2;
-      const int slot2 = p.range_kind[rg] == 1 ? 0 : 1;
-      // full register chunks, then the remainder in passes of 4, 2 and 1
-      int c0 = 0;
-      for (; c0 + CH <= per; c0 += CH)
-        rt_pass<CH, R, GEO, MIE, ATOMIC>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow,
-                                 nb, c0, flag, slot2);
-      if (CH > 4 && c0 + 4 <= per) {
-        rt_pass<4, R, GEO, MIE, ATOMIC>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
-                                c0, flag, slot2);
-        c0 += 4;
-      }
-      if (c0 + 2 <= per) {
-        rt_pass<2, R, GEO, MIE, ATOMIC>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
-                                c0, flag, slot2);
-        c0 += 2;
-      }
-      if (c0 < per)
-        rt_pass<1, R, GEO, MIE, ATOMIC>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow, nb,
-                                c0, flag, slot2);
-    }
+    if (GEO == 1 && !p.rt_cut)
+      rt_ranges<R, GEO, MIE, ATOMIC, false>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave);
+    else
+      rt_ranges<R, GEO, MIE, ATOMIC, true>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave);
 
     // ------------------------------------------ CB bins × events (cpu.jl:38-160)
     if (CB && p.compute_cb) cb_pass<R, GEO, MIE, ATOMIC>(p, ldsF, Sbins, cbrec, cbdx, slab, slabc, E, tid);
@@ -974,8 +1039,8 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
   // ---------------------------------------- energy budget and counters
   {
     double* buf = (double*)rtrec;  // reuse the record area: 5·SLOTS doubles
-    if (owner)
-      for (int c = 0; c < 5; ++c) buf[c * SLOTS + tid] = Aacc[c];
+    if (tid < SLOTS)  // slots beyond p.slots were never used: they add zero
+      for (int c = 0; c < 5; ++c) buf[c * SLOTS + tid] = owner ? Aacc[c] : 0.0;
     __syncthreads();
     if (tid < 5) {
       double sum = 0.0, cmp = 0.0;
@@ -988,8 +1053,9 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
       }
     }
   }
-  for (int c = 0; c < kCntN; ++c)
-    if (cnt[c]) atomicAdd(p.counters + c, cnt[c]);
+  if (owner)
+    for (int c = 0; c < kCntN; ++c)
+      if (cnt[c]) atomicAdd(p.counters + c, cnt[c]);
 }
 
 // One solve: workgroup g transports rays [N·g/G, N·(g+1)/G) into slab g.
@@ -1004,7 +1070,7 @@ __global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __r
   const int64_t re = p.N * ((int64_t)blockIdx.x + 1) / G;
   __syncthreads();
   run_rays<R, GEO, MIE, CB, false>(p, L, rb, re, p.slabs + (size_t)blockIdx.x * 2 * p.nout,
-                            (Slot<R>*)p.slot_mem + (size_t)blockIdx.x * Shape<R>::kSlots);
+                            slot_block<R>(p.slot_mem, blockIdx.x));
 }
 
 template <typename R, int GEO, bool MIE, bool CB>
@@ -1030,7 +1096,7 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(const BatchChunk* __res
                                                          void* slot_mem) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ int chunk_id;
-  Slot<R>* const slots = (Slot<R>*)slot_mem + (size_t)blockIdx.x * Shape<R>::kSlots;
+  Slot<R>* const slots = slot_block<R>(slot_mem, blockIdx.x);
   const void* cur = nullptr;
   for (;;) {
     if (threadIdx.x == 0) chunk_id = (int)atomicAdd(next, 1u);
@@ -1421,6 +1487,8 @@ int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_co
     for (int k = 0; k < 8; ++k) P.acs[k] = (R)(c[k] * sc);
     P.pis = (R)(3.141592653589793 * sc);
     P.tau0k = (R)(cfg->tau0 * (double)ExpK<R>::k);
+    // inside a sphere every distance to the boundary is at most the diameter
+    P.rt_cut = !(g->kind == MCRT_GEOMETRY_SPHERE && 2.0 * g->tau * (1.0 + 1e-6) < cfg->tau0);
   }
   const int Nt = (int)cfg->n_theta, Np = (int)cfg->n_phi;
   const int Ntb = cb ? (int)cfg->n_theta_b : 0, Npb = cb ? (int)cfg->n_phi_b : 0;
@@ -1514,7 +1582,7 @@ int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_co
 template <typename R>
 int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* cfg,
                 const mcrt_geometry* g, const double I0[4], uint64_t seed, uint64_t ray0,
-                uint32_t instance, const DevAlloc& alloc,
+                uint32_t instance, const mcrt_launch_shape& shape, const DevAlloc& alloc,
                 unsigned long long* dcnt, std::string* err) {
   CoopParams<R> P;
   size_t lds = 0;
@@ -1531,11 +1599,25 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
     }
     return d;
   };
-  // launch shape: LDS bounds the workgroups per CU; at most one per SLOTS rays
+  // launch shape: workgroups per CU (LDS bounds how many are resident) and
+  // active slots per workgroup; by default full slots, fewer when the solve
+  // has too few rays to give every CU a workgroup
   cp.lds_bytes = lds;
   const int wg_per_cu = std::max<int>(1, (int)(lds_max / std::max<size_t>(lds, 1)));
-  const int64_t want = (int64_t)cu_count * std::min(wg_per_cu, 2);
-  const int64_t byrays = (P.N + Shape<R>::kSlots - 1) / Shape<R>::kSlots;
+  const int per_cu = shape.workgroups_per_cu > 0 ? std::min(shape.workgroups_per_cu, 4)
+                                                 : std::min(wg_per_cu, 2);
+  int slots = Shape<R>::kSlots;
+  if (shape.slots > 0) {
+    slots = std::min<int>(Shape<R>::kSlots, std::max(64, (shape.slots + 63) / 64 * 64));
+  } else if ((P.N + slots - 1) / slots < cu_count) {
+    const int64_t per = (P.N + cu_count - 1) / cu_count;
+    slots = (int)std::min<int64_t>(Shape<R>::kSlots, std::max<int64_t>(64, (per + 63) / 64 * 64));
+  }
+  P.slots = slots;
+  cp.slots = slots;
+  cp.wg_per_cu = per_cu;
+  const int64_t want = (int64_t)cu_count * per_cu;
+  const int64_t byrays = (P.N + slots - 1) / slots;
   cp.blocks = (int)std::max<int64_t>(1, std::min(want, byrays));
   cp.threads = kThreads;
   hipError_t e;
@@ -1545,7 +1627,7 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
   cp.slabs = (double*)alloc(slab_bytes, &e);
   if (!cp.slabs) { *err = "coop: slab allocation failed"; return MCRT_ERR_NOMEM; }
   P.slabs = cp.slabs;
-  P.slot_mem = alloc(sizeof(Slot<R>) * (size_t)cp.blocks * Shape<R>::kSlots, &e);
+  P.slot_mem = alloc(slot_block_bytes<R>() * (size_t)cp.blocks, &e);
   if (!P.slot_mem) { *err = "coop: slot-state allocation failed"; return MCRT_ERR_NOMEM; }
   cp.nout = P.nout;
   cp.nrt = P.nrt;
@@ -1583,17 +1665,17 @@ hipError_t launch_r(const CoopPlan& cp, hipStream_t s) {
 
 int coop_create(CoopPlan& cp, hipStream_t, int cu_count, size_t lds_max, const mcrt_config* cfg,
                 const mcrt_geometry* geom, const double I0[4], uint64_t seed, uint64_t ray0,
-                uint32_t instance, bool mixed,
+                uint32_t instance, bool mixed, const mcrt_launch_shape& shape,
                 const DevAlloc& alloc, double* dA, double* dIrt, double* dIcb,
                 unsigned long long* dcnt, std::string* err) {
   cp.mixed = mixed;
   cp.dA = dA; cp.dIrt = dIrt; cp.dIcb = dIcb;
   if (cu_count <= 0) cu_count = 256;
   if (lds_max == 0) lds_max = 160 * 1024;
-  return mixed ? create_impl<float>(cp, cu_count, lds_max, cfg, geom, I0, seed, ray0, instance, alloc,
-                                    dcnt, err)
-               : create_impl<double>(cp, cu_count, lds_max, cfg, geom, I0, seed, ray0, instance, alloc,
-                                     dcnt, err);
+  return mixed ? create_impl<float>(cp, cu_count, lds_max, cfg, geom, I0, seed, ray0, instance, shape,
+                                    alloc, dcnt, err)
+               : create_impl<double>(cp, cu_count, lds_max, cfg, geom, I0, seed, ray0, instance, shape,
+                                     alloc, dcnt, err);
 }
 
 hipError_t coop_prepare(CoopPlan& cp, hipStream_t s) {
@@ -1631,6 +1713,7 @@ int solve_create_impl(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, cons
   int rc = init_params<R>(P, &lds, lds_max, cfg, g, I0, seed, ray0, instance, alloc, sd.counters, err,
                           cache, &sd.table_refs);
   if (rc) return rc;
+  P.slots = Shape<R>::kSlots;
   sd.mixed = sizeof(R) == 4;
   sd.geo_kind = P.geo_kind;
   sd.mie = P.sca.kind != 0;
@@ -1722,8 +1805,8 @@ hipError_t solve_finish(const SolveDev& sd, hipStream_t s) {
 }
 
 size_t batch_slot_bytes(bool mixed, int blocks) {
-  return mixed ? sizeof(Slot<float>) * (size_t)blocks * Shape<float>::kSlots
-               : sizeof(Slot<double>) * (size_t)blocks * Shape<double>::kSlots;
+  return mixed ? slot_block_bytes<float>() * (size_t)blocks
+               : slot_block_bytes<double>() * (size_t)blocks;
 }
 
 hipError_t batch_set_lds(bool mixed, size_t lds_bytes) {

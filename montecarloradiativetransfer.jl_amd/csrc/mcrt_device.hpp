@@ -16,6 +16,7 @@
 namespace mcrt {
 
 #define MCRT_DEV __device__ __forceinline__
+#define MCRT_HD __host__ __device__ __forceinline__
 
 // ---------------------------------------------------------------- math shims
 template <typename R> struct M;

@@ -138,6 +138,10 @@ class mcrt_request(C.Structure):
     ]
 
 
+class mcrt_launch_shape(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("workgroups_per_cu", C.c_int32)]
+
+
 class mcrt_server_stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "cancelled", "failed",
                                           "launches", "chunks", "rays", "queued", "running")] + [
@@ -180,6 +184,10 @@ SIGNATURES = {
                                          C.c_uint64, C.c_int32, C.c_uint64, C.c_uint32,
                                          C.POINTER(_vp)]),
     "mcrt_plan_read_sigma": (C.c_int, [_vp, _dp, _dp, _dp, C.POINTER(C.c_int32)]),
+    "mcrt_plan_create_shaped": (C.c_int, [_vp, C.POINTER(mcrt_config), C.POINTER(mcrt_geometry), _dp,
+                                          C.c_uint64, C.c_int32, C.c_uint64, C.c_uint32,
+                                          C.POINTER(mcrt_launch_shape), C.POINTER(_vp)]),
+    "mcrt_plan_shape": (C.c_int, [_vp, C.POINTER(mcrt_launch_shape), C.POINTER(C.c_int32)]),
     "mcrt_batch_create": (C.c_int, [_vp, C.c_int32, C.POINTER(mcrt_request), C.POINTER(_vp)]),
     "mcrt_batch_launch": (C.c_int, [_vp]),
     "mcrt_batch_wait": (C.c_int, [_vp]),

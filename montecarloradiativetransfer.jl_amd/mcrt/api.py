@@ -118,7 +118,8 @@ class Plan:
     """A resident solve: inputs staged in HBM once, launched any number of times."""
 
     def __init__(self, cfg: Config, g: AbstractGeometry, I0, *, seed: int = DEFAULT_SEED,
-                 precision: str = "fp64", device: int = 0, ray_begin: int = 0, instance: int = 0):
+                 precision: str = "fp64", device: int = 0, ray_begin: int = 0, instance: int = 0,
+                 shape: tuple | None = None):
         g.validate()
         self.cfg = cfg
         self.ctx = get_context(device)
@@ -126,12 +127,22 @@ class Plan:
         h = C.c_void_p()
         if ray_begin < 0 or not 0 <= instance < 2**32:
             raise ValueError("ray_begin must be >= 0 and instance a 32-bit unsigned integer")
-        rc = self.ctx.lib.mcrt_plan_create_range(self.ctx.handle, C.byref(cs), C.byref(gs),
-                                                 _abi.dptr(i0), C.c_uint64(seed & (2**64 - 1)),
-                                                 _abi.PRECISIONS[precision], C.c_uint64(ray_begin),
-                                                 C.c_uint32(instance), C.byref(h))
+        sh = _abi.mcrt_launch_shape(*(shape or (0, 0)))
+        rc = self.ctx.lib.mcrt_plan_create_shaped(self.ctx.handle, C.byref(cs), C.byref(gs),
+                                                  _abi.dptr(i0), C.c_uint64(seed & (2**64 - 1)),
+                                                  _abi.PRECISIONS[precision], C.c_uint64(ray_begin),
+                                                  C.c_uint32(instance), C.byref(sh), C.byref(h))
         _abi.check(rc, self.ctx.handle)
         self.handle = h
+
+    def shape(self) -> dict:
+        """Launch shape: slots per workgroup, workgroups per CU, grid size."""
+        sh = _abi.mcrt_launch_shape()
+        wg = C.c_int32()
+        _abi.check(self.ctx.lib.mcrt_plan_shape(self.handle, C.byref(sh), C.byref(wg)),
+                   self.ctx.handle)
+        return {"slots": sh.slots, "workgroups_per_cu": sh.workgroups_per_cu,
+                "workgroups": wg.value}
 
     def launch(self) -> None:
         _abi.check(self.ctx.lib.mcrt_plan_launch(self.handle), self.ctx.handle)

@@ -8,6 +8,7 @@
                         bins × 48 azimuths; CB θb = 0:0.05°:5° (101) × 8; seed 7
   plane_rayleigh_cb     plane_rayleigh_tau1 with CB on (default θb grid), 1e9 rays:
                         the mixed-vs-FP64 precision case
+  sphere_mie_rt         sphere_mie_cb without CB
 The ray counts of the full workloads are 1e6, 1e7 and 1e9; tests use fewer rays
 of the same configuration.
 """
@@ -69,5 +70,10 @@ def plane_rayleigh_cb(n_rays: int = 10**9):
     return plane_rayleigh_tau1(n_rays, compute_cb=True)
 
 
+def sphere_mie_rt(n_rays: int = 10**7):
+    """sphere_mie_cb without CB (the peel-off alone)."""
+    return sphere_mie_cb(n_rays, compute_cb=False)
+
+
 WORKLOADS = {"plane_rayleigh_tau1": plane_rayleigh_tau1, "sphere_mie_cb": sphere_mie_cb,
-             "plane_rayleigh_cb": plane_rayleigh_cb}
+             "plane_rayleigh_cb": plane_rayleigh_cb, "sphere_mie_rt": sphere_mie_rt}

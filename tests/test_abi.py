@@ -44,7 +44,8 @@ def test_struct_layout_matches_c(tmp_path):
     structs = {"mcrt_scatterer": _abi.mcrt_scatterer, "mcrt_layer": _abi.mcrt_layer,
                "mcrt_geometry": _abi.mcrt_geometry, "mcrt_config": _abi.mcrt_config,
                "mcrt_stats": _abi.mcrt_stats, "mcrt_request": _abi.mcrt_request,
-               "mcrt_server_stats": _abi.mcrt_server_stats}
+               "mcrt_server_stats": _abi.mcrt_server_stats,
+               "mcrt_launch_shape": _abi.mcrt_launch_shape}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for sname, cls in structs.items():
         lines.append(f'printf("{sname} %zu\\n", sizeof({sname}));')

@@ -109,3 +109,24 @@ def test_random_population_runs():
             assert st["rays"] == r.cfg.number_of_rays
             assert np.all(np.isfinite(out))
             assert abs(sum(out[:5]) - 1.0) < 1e-5
+
+
+def test_launch_shapes_and_autotuner():
+    from mcrt.api import Plan
+    from mcrt.tune import autotune
+    cfg, g, seed = W.sphere_mie_cb(32768, Ntheta_table=1801)
+    ref = flat(mcrt.simulate(cfg, g, I0, seed=seed, precision="mixed"))
+    for shape in [(64, 1), (128, 2), (256, 1)]:
+        p = Plan(cfg, g, I0, seed=seed, precision="mixed", shape=shape)
+        info = p.shape()
+        assert info["slots"] == shape[0] and info["workgroups_per_cu"] == shape[1]
+        p.launch()
+        out = flat(p.read())
+        p.close()
+        err = float(np.max(np.abs(out - ref)))
+        print(f"[shape {shape}] workgroups={info['workgroups']} max|Δ|={err:.3e}", flush=True)
+        assert err <= 1e-5 * float(np.max(np.abs(ref)))  # same rays, FP32 grouping only
+    res = autotune(cfg, g, I0, precision="mixed", sample_rays=32768, cache={})
+    print(f"[autotune] {res['shape']} {res['rays_per_s']:.4g} rays/s", flush=True)
+    assert len(res["trials"]) == 8 and res["shape"] in [(t["slots"], t["workgroups_per_cu"])
+                                                         for t in res["trials"]]
