@@ -6,7 +6,8 @@ history a rounding flips), not Monte Carlo noise.  The report states that
 difference per output bin, absolutely and in units of the bin's Monte Carlo
 standard error σ (batch means of the FP64 solve), for the energy budget A, the
 peel-off Stokes components I, Q, U, V, the CB Stokes components and the CB
-enhancement ζ(θb) = ⟨Icb_I(θb, ·)⟩_ϕ / ⟨Icb_I(θb_ref, ·)⟩_ϕ.
+enhancement ζ(θb) = ⟨Icb_I(θb, ·)⟩_ϕ / ⟨Icb_I(θb_ref, ·)⟩_ϕ, θb_ref the largest θb up to 60°
+with a non-zero CB intensity (or a chosen angle).
 
 Large solves run as consecutive ray ranges of the stream (shards), merged by
 ray-weighted means; the union is exactly the rays of one solve of that size.
@@ -116,7 +117,10 @@ def compare_solves(cfg, fp64, mixed, *, ref_theta_b_deg=None):
             rep[f"Icb_{name}"] = _group_stats(Icb32[c], Icb64[c], sIcb[c])
         thb = np.degrees(np.arccos(np.clip(np.asarray(cfg.costhetab, float), -1, 1)))
         if ref_theta_b_deg is None:
-            ok = np.nonzero(thb <= 90.0 + 1e-9)[0]
+            # the largest θb up to 60° whose CB intensity is non-zero (towards 90°
+            # the backscattered rays graze the slab and carry almost nothing)
+            mean = np.asarray(Icb64[0]).mean(axis=1)
+            ok = np.nonzero((thb <= 60.0 + 1e-9) & (mean > 0))[0]
             ref_index = int(ok[-1]) if ok.size else len(thb) - 1
         else:
             ref_index = int(np.argmin(np.abs(thb - ref_theta_b_deg)))
