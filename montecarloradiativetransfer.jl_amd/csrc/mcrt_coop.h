@@ -24,7 +24,9 @@ struct CoopPlan {
   int nslab = 0;               // workgroup partial-bin slabs
   int64_t slab_stride = 0;     // doubles per slab (sum + compensation)
   double* slabs = nullptr;
-  int64_t nout = 0;            // 5 + nrt + ncb
+  int64_t nout = 0;            // outputs: 5 + nrt + ncb
+  int64_t slab_words = 0;      // words of one slab half: 5 + nrt + cb_groups·ncb
+  int cb_groups = 1;
   double* dA = nullptr;
   double* dIrt = nullptr;
   double* dIcb = nullptr;

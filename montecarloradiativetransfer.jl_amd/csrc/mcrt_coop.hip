@@ -102,7 +102,8 @@ template <typename R> struct CoopParams {
   uint32_t instance;
   void* slot_mem;         // per workgroup: SLOTS Slot<R>, then SLOTS SlotAcc
   double* slabs;          // [blocks][2][nout]: Kahan sum, compensation
-  int64_t nout;           // 5 + nrt + ncb
+  int64_t nout;           // slab words: 5 + nrt + cb_groups·ncb (private slabs) or 5 + nrt + ncb
+  int cb_groups;          // CB work items per bin: events e ≡ g (mod cb_groups)
   int64_t nrt, ncb;
   unsigned long long* counters;
 };
@@ -532,13 +533,21 @@ MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F, const R* rtrec, R* e
 
 // CB bins × events (cpu.jl:83-158): thread tid owns CB bins tid, tid + kThreads, ….
 // CB bin: kf[0..2], Eh[3..4], Eh × kf [5..7], cos2ψ_b [8], sin2ψ_b [9], 1/kf_z [10].
+// Work items are (bin, group) pairs, item = lb + nb·g: the events of a bin are
+// split into cb_groups interleaved groups so that the items spread evenly over
+// the workgroup (808 bins would leave 216 of 512 threads with half the work).
+// A private slab holds one copy of the CB bins per group; shared slabs take
+// every group with atomics.
 template <typename R, int GEO, bool MIE, bool ATOMIC>
 MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* cbrec,
                       const double* cbdx, double* slab, double* slabc, int E, int tid) {
   const int nb = p.Ntb * p.Npb;
+  // split only iterations with enough events to share (uniform over the workgroup)
+  const int G = E >= 8 * p.cb_groups ? p.cb_groups : 1;
   const R tau0 = p.tau0;
   const R tau2 = p.tau[0] * p.tau[0];
-  for (int lb = tid; lb < nb; lb += kThreads) {
+  for (int it = tid; it < nb * G; it += kThreads) {
+    const int lb = it % nb, grp = it / nb;
     R b[kBinW];
     const R* src = p.cb_bin + kBinStride * lb;
 #pragma unroll
@@ -547,7 +556,7 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
     const double* kk = p.cb_kik + 4 * lb;
     const double kikx = kk[0], kiky = kk[1], kikz = kk[2];
     R acc[4] = {R(0), R(0), R(0), R(0)};
-    for (int e = 0; e < E; ++e) {
+    for (int e = grp; e < E; e += G) {
       const R* cr = cbrec + kCbRec * e;
       const int kind = (int)cr[39];
       if (kind == 0) continue;
@@ -605,7 +614,7 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[c] += o[c];
     }
-    const int64_t ob = 5 + p.nrt + 4 * (int64_t)lb;
+    const int64_t ob = 5 + p.nrt + (ATOMIC ? 0 : (int64_t)grp * p.ncb) + 4 * (int64_t)lb;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       if (ATOMIC) unsafeAtomicAdd(slab + ob + c, (double)acc[c]);
@@ -1129,17 +1138,28 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(const BatchChunk* __res
   }
 }
 
-// out[i] = Σ_g (sum_g[i] - comp_g[i]) in workgroup order (Kahan).
-__global__ void slab_reduce(const double* __restrict__ slabs, int nslab, int64_t nout,
+// The value of output i in one private slab: output words A and Irt map
+// one-to-one, a CB word is the sum over the slab's CB groups.
+MCRT_DEV double slab_value(const double* sl, int64_t sn, int64_t i, int64_t nrt, int64_t ncb,
+                           int groups) {
+  if (i < 5 + nrt) return sl[i] - sl[sn + i];
+  double v = 0.0;
+  for (int q = 0; q < groups; ++q) {
+    const int64_t j = i + (int64_t)q * ncb;
+    v += sl[j] - sl[sn + j];
+  }
+  return v;
+}
+
+// out[i] = Σ_g slab_value_g(i) in workgroup order (Kahan); sn = slab words.
+__global__ void slab_reduce(const double* __restrict__ slabs, int nslab, int64_t sn, int64_t nout,
                             double* __restrict__ A, double* __restrict__ Irt, double* __restrict__ Icb,
-                            int64_t nrt) {
+                            int64_t nrt, int64_t ncb, int groups) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nout) return;
   double s = 0.0, c = 0.0;
-  for (int g = 0; g < nslab; ++g) {
-    const double* sl = slabs + (size_t)g * 2 * nout;
-    kahan(&s, &c, sl[i] - sl[nout + i]);
-  }
+  for (int g = 0; g < nslab; ++g)
+    kahan(&s, &c, slab_value(slabs + (size_t)g * 2 * sn, sn, i, nrt, ncb, groups));
   if (i < 5) A[i] = s;
   else if (i < 5 + nrt) Irt[i - 5] = s;
   else Icb[i - 5 - nrt] = s;
@@ -1149,22 +1169,19 @@ __global__ void slab_reduce(const double* __restrict__ slabs, int nslab, int64_t
 // workgroup ray ranges: with S_g the raw sum of workgroup g over its n_g rays,
 // v = Σ_g (S_g - n_g·T/N)² / n_g / (G - 1) estimates the per-ray variance and
 // σ = √(v/N).  NaN when G < 2.
-__global__ void slab_sigma(const double* __restrict__ slabs, int nslab, int64_t nout, int64_t N,
-                           double* __restrict__ sig) {
+__global__ void slab_sigma(const double* __restrict__ slabs, int nslab, int64_t sn, int64_t nout,
+                           int64_t N, int64_t nrt, int64_t ncb, int groups, double* __restrict__ sig) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nout) return;
   if (nslab < 2) { sig[i] = NAN; return; }
   double T = 0.0, c = 0.0;
-  for (int g = 0; g < nslab; ++g) {
-    const double* sl = slabs + (size_t)g * 2 * nout;
-    kahan(&T, &c, sl[i] - sl[nout + i]);
-  }
+  for (int g = 0; g < nslab; ++g)
+    kahan(&T, &c, slab_value(slabs + (size_t)g * 2 * sn, sn, i, nrt, ncb, groups));
   const double mean = T / (double)N;
   double v = 0.0;
   for (int g = 0; g < nslab; ++g) {
-    const double* sl = slabs + (size_t)g * 2 * nout;
     const int64_t ng = N * (g + 1) / nslab - N * g / nslab;  // the kernel's ray range
-    const double d = (sl[i] - sl[nout + i]) - (double)ng * mean;
+    const double d = slab_value(slabs + (size_t)g * 2 * sn, sn, i, nrt, ncb, groups) - (double)ng * mean;
     v += d * d / (double)ng;
   }
   sig[i] = sqrt(v / (double)(nslab - 1) / (double)N);
@@ -1574,6 +1591,17 @@ int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_co
   P.nrt = 4 * (int64_t)Nt * Np;
   P.ncb = 4 * (int64_t)cfg->n_theta_b * cfg->n_phi_b;
   P.nout = 5 + P.nrt + P.ncb;
+  {
+    // the fewest groups g ≤ 8 minimising ⌈g·nb/threads⌉/g (the CB pass time)
+    const int nb = Ntb * Npb;
+    int best = 1;
+    double tbest = 1e300;
+    for (int g = 1; g <= 8 && nb > 0; ++g) {
+      const double t = (double)((g * nb + kThreads - 1) / kThreads) / g;
+      if (t < tbest - 1e-12) { tbest = t; best = g; }
+    }
+    P.cb_groups = best;
+  }
   P.counters = dcnt;
   *lds_out = lds;
   return MCRT_OK;
@@ -1621,6 +1649,9 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
   cp.blocks = (int)std::max<int64_t>(1, std::min(want, byrays));
   cp.threads = kThreads;
   hipError_t e;
+  P.nout = 5 + P.nrt + (int64_t)P.cb_groups * P.ncb;  // private slabs: a CB copy per group
+  cp.cb_groups = P.cb_groups;
+  cp.slab_words = P.nout;
   cp.nslab = cp.blocks;
   cp.slab_stride = 2 * P.nout;
   size_t slab_bytes = sizeof(double) * (size_t)cp.nslab * (size_t)cp.slab_stride;
@@ -1629,7 +1660,7 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
   P.slabs = cp.slabs;
   P.slot_mem = alloc(slot_block_bytes<R>() * (size_t)cp.blocks, &e);
   if (!P.slot_mem) { *err = "coop: slot-state allocation failed"; return MCRT_ERR_NOMEM; }
-  cp.nout = P.nout;
+  cp.nout = 5 + P.nrt + P.ncb;
   cp.nrt = P.nrt;
   cp.ncb = P.ncb;
   cp.params_dev = up(&P, sizeof P);
@@ -1831,7 +1862,7 @@ hipError_t coop_sigma(CoopPlan& cp, hipStream_t s, int64_t N, double* sig) {
   const int th = 256;
   const int bl = (int)((cp.nout + th - 1) / th);
   hipLaunchKernelGGL(slab_sigma, dim3(bl), dim3(th), 0, s, (const double*)cp.slabs, cp.nslab,
-                     cp.nout, N, sig);
+                     cp.slab_words, cp.nout, N, cp.nrt, cp.ncb, cp.cb_groups, sig);
   return hipGetLastError();
 }
 
@@ -1839,7 +1870,7 @@ hipError_t coop_reduce(CoopPlan& cp, hipStream_t s) {
   const int th = 256;
   const int bl = (int)((cp.nout + th - 1) / th);
   hipLaunchKernelGGL(slab_reduce, dim3(bl), dim3(th), 0, s, (const double*)cp.slabs, cp.nslab,
-                     cp.nout, cp.dA, cp.dIrt, cp.dIcb, cp.nrt);
+                     cp.slab_words, cp.nout, cp.dA, cp.dIrt, cp.dIcb, cp.nrt, cp.ncb, cp.cb_groups);
   return hipGetLastError();
 }
 

@@ -110,6 +110,62 @@ void own(const mcrt_request& q, OwnedRequest& o) {
   o.req.geom = &o.geom;
 }
 
+// Memory kept for reuse in power-of-two size classes (device, or pinned host),
+// so that admitting and retiring a solve calls no allocator in steady state —
+// and no hipFree, which waits for the device.
+class Pool {
+ public:
+  Pool(bool host, size_t keep) : host_(host), keep_(keep) {}
+  ~Pool() {
+    for (auto& kv : free_)
+      for (void* p : kv.second) release_raw(p);
+  }
+  static size_t size_class(size_t n) {
+    size_t c = 256;
+    while (c < n) c <<= 1;
+    return c;
+  }
+  void* get(size_t n, hipError_t* e) {
+    const size_t c = size_class(n);
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto it = free_.find(c);
+      if (it != free_.end() && !it->second.empty()) {
+        void* p = it->second.back();
+        it->second.pop_back();
+        cached_ -= c;
+        *e = hipSuccess;
+        return p;
+      }
+    }
+    void* p = nullptr;
+    *e = host_ ? hipHostMalloc(&p, c, hipHostMallocDefault) : hipMalloc(&p, c);
+    return *e == hipSuccess ? p : nullptr;
+  }
+  void put(void* p, size_t n) {
+    if (!p) return;
+    const size_t c = size_class(n);
+    std::lock_guard<std::mutex> g(mu_);
+    free_[c].push_back(p);
+    cached_ += c;
+    // beyond the limit, free the largest classes first
+    for (auto it = free_.rbegin(); cached_ > keep_ && it != free_.rend(); ++it)
+      while (cached_ > keep_ && !it->second.empty()) {
+        release_raw(it->second.back());
+        it->second.pop_back();
+        cached_ -= it->first;
+      }
+  }
+
+ private:
+  void release_raw(void* p) { (void)(host_ ? hipHostFree(p) : hipFree(p)); }
+  bool host_;
+  size_t keep_;
+  size_t cached_ = 0;
+  std::mutex mu_;
+  std::map<size_t, std::vector<void*>> free_;
+};
+
 struct Ticket {
   uint64_t id = 0;
   OwnedRequest r;
@@ -121,7 +177,7 @@ struct Ticket {
   size_t need = 0;
   bool admitted = false;  // holds `need` bytes of the budget
   mcrt::SolveDev sd;
-  std::vector<void*> allocs;
+  std::vector<std::pair<void*, size_t>> allocs;  // from the server's device pool
   int64_t next_chunk = 0;
   int inflight = 0;           // launches still running with chunks of this solve
   bool finished = false;      // the launch with its reduction and read-back completed
@@ -149,7 +205,10 @@ struct mcrt_server {
   uint64_t next_id = 1;
   size_t budget = 0, in_use = 0;
   mcrt::TableCache* cache = nullptr;
+  Pool* dev_pool = nullptr;
+  Pool* host_pool = nullptr;
   hipStream_t setup = nullptr;
+  hipEvent_t setup_ev = nullptr;  // after the last admitted solve's clear
   struct Lane {
     hipStream_t s = nullptr;
     void* slot_mem = nullptr;
@@ -176,7 +235,7 @@ void mcrt_server::release(Ticket* t) {
   if (t->released) return;
   mcrt::table_cache_release(cache, t->sd.table_refs);
   t->sd.table_refs.clear();
-  for (void* p : t->allocs) (void)hipFree(p);
+  for (auto& a : t->allocs) dev_pool->put(a.first, a.second);
   t->allocs.clear();
   t->released = true;
   if (t->admitted) {
@@ -189,11 +248,11 @@ void mcrt_server::release(Ticket* t) {
 // Create the device state of newly admitted solves (outside the lock).
 void mcrt_server::admit(std::vector<Ticket*>& batch) {
   for (Ticket* t : batch) {
-    auto alloc = [t](size_t n, hipError_t* e) -> void* {
-      void* p = nullptr;
-      *e = hipMalloc(&p, n ? n : 8);
-      if (*e != hipSuccess) return nullptr;
-      t->allocs.push_back(p);
+    Pool* pool = dev_pool;
+    auto alloc = [t, pool](size_t n, hipError_t* e) -> void* {
+      void* p = pool->get(n ? n : 8, e);
+      if (!p) return nullptr;
+      t->allocs.push_back({p, n ? n : 8});
       return p;
     };
     std::string err;
@@ -201,14 +260,14 @@ void mcrt_server::admit(std::vector<Ticket*>& batch) {
     int rc = mcrt::solve_create(t->sd, ctx->lds_per_block, q.cfg, q.geom, q.I0, q.seed, q.ray_begin,
                                 q.instance, t->mixed, q.chunk_rays, alloc, &err, cache);
     hipError_t e = hipSuccess;
+    // the clear runs on the set-up stream; launches wait for setup_ev
     if (!rc) e = mcrt::solve_prepare(t->sd, setup);
-    if (!rc && e == hipSuccess) e = hipStreamSynchronize(setup);
     if (!rc && e == hipSuccess) {
       const size_t n = (size_t)t->sd.nout;
-      e = hipHostMalloc((void**)&t->h_out, sizeof(double) * n, hipHostMallocDefault);
-      if (e == hipSuccess) e = hipHostMalloc((void**)&t->h_sig, sizeof(double) * n, hipHostMallocDefault);
+      t->h_out = (double*)host_pool->get(sizeof(double) * n, &e);
+      if (e == hipSuccess) t->h_sig = (double*)host_pool->get(sizeof(double) * n, &e);
       if (e == hipSuccess)
-        e = hipHostMalloc((void**)&t->h_cnt, sizeof(unsigned long long) * mcrt::kCntN, hipHostMallocDefault);
+        t->h_cnt = (unsigned long long*)host_pool->get(sizeof(unsigned long long) * mcrt::kCntN, &e);
     }
     if (!rc && e != hipSuccess) {
       rc = e == hipErrorOutOfMemory ? MCRT_ERR_NOMEM : MCRT_ERR_HIP;
@@ -230,6 +289,7 @@ void mcrt_server::admit(std::vector<Ticket*>& batch) {
     t->t_start = Clock::now();
     active.push_back(t);
   }
+  (void)hipEventRecord(setup_ev, setup);
 }
 
 // Fill and launch one window on lane L; false when there is nothing to run.
@@ -247,8 +307,8 @@ bool mcrt_server::launch(Lane& L) {
     for (Ticket* t : active)
       if (!t->cancel && t->mixed == mixed && t->next_chunk < t->sd.nchunks) elig.push_back(t);
   }
-  // round-robin over the solves (oldest first) up to ~4 chunks per workgroup
-  const int target = std::min(kLaneChunks, std::max(4 * blocks, (int)elig.size()));
+  // round-robin over the solves (oldest first) up to ~2 chunks per workgroup
+  const int target = std::min(kLaneChunks, std::max(2 * blocks, (int)elig.size()));
   struct Item { double cost; Ticket* t; int64_t k; };
   std::vector<Item> items;
   std::vector<int64_t> take(elig.size(), 0);
@@ -280,6 +340,7 @@ bool mcrt_server::launch(Lane& L) {
   for (size_t j = 0; j < items.size(); ++j) mcrt::solve_chunk(items[j].t->sd, items[j].k, &L.hchunks[j]);
   hipError_t e = hipMemcpyAsync(L.dchunks, L.hchunks, sizeof(mcrt::BatchChunk) * items.size(),
                                 hipMemcpyHostToDevice, L.s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(L.s, setup_ev, 0);  // the solves' slabs are clear
   if (e == hipSuccess)
     e = mcrt::batch_launch(mixed, L.dchunks, (int)items.size(), L.dnext, L.slot_mem, blocks, lds, L.s);
   // a solve finishing here may still have chunks in the other lane's launch:
@@ -447,8 +508,11 @@ extern "C" int mcrt_server_create(mcrt_ctx* ctx, int64_t device_budget_bytes,
   S->budget = device_budget_bytes > 0 ? (size_t)device_budget_bytes : free_b / 2;
   S->cache = mcrt::table_cache_create(table_cache_bytes > 0 ? (size_t)table_cache_bytes : (size_t)1 << 30);
   S->blocks = std::max(1, ctx->cu_count);
+  S->dev_pool = new Pool(false, (size_t)4 << 30);
+  S->host_pool = new Pool(true, (size_t)1 << 30);
   const size_t slot = std::max(mcrt::batch_slot_bytes(true, S->blocks), mcrt::batch_slot_bytes(false, S->blocks));
   e = hipStreamCreateWithFlags(&S->setup, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&S->setup_ev, hipEventDisableTiming);
   for (auto& L : S->lanes) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&L.slot_mem, slot);
@@ -568,9 +632,10 @@ extern "C" int mcrt_result(mcrt_server* S, uint64_t ticket, double A[5], double*
               (unsigned long long)ticket, t->err.c_str());
   }
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(t->t_done - t->t_submit).count();
-  if (t->h_out) (void)hipHostFree(t->h_out);
-  if (t->h_sig) (void)hipHostFree(t->h_sig);
-  if (t->h_cnt) (void)hipHostFree(t->h_cnt);
+  const size_t n = (size_t)t->sd.nout;
+  S->host_pool->put(t->h_out, sizeof(double) * n);
+  S->host_pool->put(t->h_sig, sizeof(double) * n);
+  S->host_pool->put(t->h_cnt, sizeof(unsigned long long) * mcrt::kCntN);
   delete t;
   return rc;
 }
@@ -617,9 +682,10 @@ extern "C" void mcrt_server_destroy(mcrt_server* S) {
   for (auto& kv : S->tickets) {
     Ticket* t = kv.second;
     S->release(t);
-    if (t->h_out) (void)hipHostFree(t->h_out);
-    if (t->h_sig) (void)hipHostFree(t->h_sig);
-    if (t->h_cnt) (void)hipHostFree(t->h_cnt);
+    const size_t n = (size_t)t->sd.nout;
+    S->host_pool->put(t->h_out, sizeof(double) * n);
+    S->host_pool->put(t->h_sig, sizeof(double) * n);
+    S->host_pool->put(t->h_cnt, sizeof(unsigned long long) * mcrt::kCntN);
     delete t;
   }
   for (auto& L : S->lanes) {
@@ -631,7 +697,11 @@ extern "C" void mcrt_server_destroy(mcrt_server* S) {
     if (L.slot_mem) (void)hipFree(L.slot_mem);
     if (L.s) (void)hipStreamDestroy(L.s);
   }
+  if (S->setup) (void)hipStreamSynchronize(S->setup);
+  if (S->setup_ev) (void)hipEventDestroy(S->setup_ev);
   if (S->setup) (void)hipStreamDestroy(S->setup);
   mcrt::table_cache_destroy(S->cache);
+  delete S->dev_pool;
+  delete S->host_pool;
   delete S;
 }
