@@ -211,6 +211,24 @@ int mcrt_plan_create(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geometry*
 int mcrt_plan_create_range(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geometry* geom,
                            const double I0[4], uint64_t seed, int32_t precision,
                            uint64_t ray_begin, uint32_t instance, mcrt_plan** out);
+
+/* Launch shape of a cooperative plan (what the autotuner chooses): ray slots
+ * per 512-thread workgroup (a multiple of 64; at most 256 mixed, 128 fp64) and
+ * workgroups launched per CU (1..4).  0 selects the default: full slots (fewer
+ * when N is too small to give every CU a workgroup) and as many workgroups
+ * per CU as LDS allows, at most 2.  The plain form ignores it.  No reference
+ * counterpart. */
+typedef struct mcrt_launch_shape {
+  int32_t slots;
+  int32_t workgroups_per_cu;
+} mcrt_launch_shape;
+
+int mcrt_plan_create_shaped(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geometry* geom,
+                            const double I0[4], uint64_t seed, int32_t precision,
+                            uint64_t ray_begin, uint32_t instance, const mcrt_launch_shape* shape,
+                            mcrt_plan** out);
+/* The shape a plan was built with (workgroups: the launch's grid). */
+int mcrt_plan_shape(mcrt_plan* plan, mcrt_launch_shape* shape, int32_t* workgroups);
 int mcrt_plan_launch(mcrt_plan* plan);   /* asynchronous on the context stream */
 int mcrt_plan_wait(mcrt_plan* plan);     /* blocks until the launch finished */
 int mcrt_plan_read(mcrt_plan* plan, double A[5], double* Irt, double* Icb,
