@@ -137,6 +137,10 @@ typedef struct mcrt_stats {
   uint64_t nonfinite;    /* rays whose weight became NaN/Inf */
   double kernel_ms;      /* device time of the transport launch(es) */
   double total_ms;       /* wall time of the call */
+  uint64_t batches;      /* batch means behind the standard errors: workgroup
+                            slabs of a plan, batch slabs of a batched or served
+                            solve (0: the plain form, which has none) */
+  uint64_t chunks;       /* work chunks of a batched or served solve (0: plan) */
 } mcrt_stats;
 
 typedef struct mcrt_ctx mcrt_ctx;
@@ -259,7 +263,10 @@ typedef struct mcrt_request {
   uint64_t ray_begin;   /* rays ray_begin .. ray_begin+N-1 of the stream */
   uint32_t instance;
   int32_t precision;    /* MCRT_PRECISION_FP64 or MCRT_PRECISION_MIXED */
-  int64_t chunk_rays;   /* rays per work chunk; 0: the default for N */
+  int64_t chunk_rays;   /* rays per work chunk; 0: the library's default for
+                           this solve (about 4e7 bin evaluations per chunk from
+                           the solve's bins and a model of its events per ray,
+                           64..8192 rays, at least two chunks when N >= 2) */
 } mcrt_request;
 
 typedef struct mcrt_batch mcrt_batch;

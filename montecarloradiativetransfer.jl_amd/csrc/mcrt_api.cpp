@@ -528,6 +528,8 @@ extern "C" int mcrt_plan_read(mcrt_plan* P, double A[5], double* Irt, double* Ic
     mcrt_plan_kernel_ms(P, &ms);
     st->kernel_ms = ms;
     st->total_ms = P->total_ms;
+    st->batches = P->precision == MCRT_PRECISION_FP64_PLAIN ? 0 : (uint64_t)P->coop.nslab;
+    st->chunks = 0;
   }
   return MCRT_OK;
 }

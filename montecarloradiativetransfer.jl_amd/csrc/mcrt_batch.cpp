@@ -226,6 +226,8 @@ extern "C" int mcrt_batch_read(mcrt_batch* B, int32_t i, double A[5], double* Ir
     mcrt_batch_kernel_ms(B, &ms);
     st->kernel_ms = ms;
     st->total_ms = ms;
+    st->batches = (uint64_t)sd.nbatch;
+    st->chunks = (uint64_t)sd.nchunks;
   }
   return MCRT_OK;
 }

@@ -76,17 +76,20 @@ struct SolveDev {
   std::vector<const void*> table_refs;     // cached tables in use (TableCache)
 };
 
-// Rays per chunk when the caller does not choose: between 512 and 8192, at
-// least two chunks when N ≥ 2.  A function of N alone, so that a solve is cut
-// the same way alone and in any batch.
-int64_t default_chunk_rays(int64_t N);
+// Rays per chunk when the caller does not choose: about a fixed number of bin
+// evaluations per chunk from solve_work_per_ray, between 64 and 8192 rays, at
+// least two chunks when N ≥ 2.  A function of the solve alone, so that a
+// solve is cut the same way alone and in any batch.
+double solve_work_per_ray(const mcrt_config* cfg, const mcrt_geometry* g);
+int64_t default_chunk_rays(int64_t N, double work_per_ray);
 // Tables go to memory from alloc, or are shared through cache when given.
 int solve_create(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, const mcrt_geometry* geom,
                  const double I0[4], uint64_t seed, uint64_t ray0, uint32_t instance, bool mixed,
                  int64_t chunk_rays, const DevAlloc& alloc, std::string* err,
                  TableCache* cache = nullptr);
 // Device bytes of a solve apart from cached tables (for admission).
-size_t solve_bytes_estimate(const mcrt_config* cfg, bool mixed, int64_t chunk_rays);
+size_t solve_bytes_estimate(const mcrt_config* cfg, const mcrt_geometry* g, bool mixed,
+                            int64_t chunk_rays);
 void solve_chunk(const SolveDev& sd, int64_t k, BatchChunk* out);
 // Zero the batch slabs and counters (kernels, so the sequence can be captured).
 hipError_t solve_prepare(const SolveDev& sd, hipStream_t stream);

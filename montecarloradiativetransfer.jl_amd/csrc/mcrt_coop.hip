@@ -18,8 +18,11 @@
 //    back into each ray's weight are wave-reduced per (event, bin chunk) and
 //    summed over waves in a fixed order.
 //  * Bins are flushed after each iteration into the workgroup's private FP64
-//    slab with Kahan compensation; one reduction kernel sums the slabs in
-//    workgroup order.  Results are bitwise reproducible for a fixed launch.
+//    slab by atomic adds that return nothing (a slab takes at most ~10⁴
+//    iteration partials, so its rounding stays near 1e-12 relative, far below
+//    that of an FP32 partial); one reduction kernel sums the slabs in
+//    workgroup order with Kahan compensation.  Results are bitwise
+//    reproducible for a fixed launch.
 //  * Mie F/csrn tables (and S in FP32) are staged in LDS, padded by one zero
 //    node so that the interpolation needs no end-of-table branch.
 //  * CB uses O(1) path state per ray in FP64: the forward field of
@@ -415,9 +418,9 @@ template <typename R> MCRT_DEV Slot<R>* slot_block(void* base, int block) {
 // sum of each event is wave-reduced and added by lane 0 to esc in chunk order.
 // RT record: m[0..7], I[8..11], J[12..13], K·x[14..16], [17] K·τ of the layer
 // (plane) or K²·(τ² - x·x) (sphere), [18] flags; K = ExpK<R>::k.
-template <int CH, typename R, int GEO, bool MIE, bool ATOMIC, bool CUT>
+template <int CH, typename R, int GEO, bool MIE, bool CUT>
 MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc, double* slab,
-                      double* slabc, int E, int tid, int lane, int wave, int jb, int nrow, int nb,
+                      int E, int tid, int lane, int wave, int jb, int nrow, int nb,
                       int c0, int flag, int slot2) {
   R bk[CH][kBinW];
   int bidx[CH];
@@ -484,25 +487,24 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
     part = wave_sum(part);
     if (lane == 0) esc[(e * kWaves + wave) * 2 + slot2] += part;
   }
-  // flush into the slab: Kahan into a workgroup-private slab (bins are owned
-  // by one thread: no races), or FP64 atomics into a slab shared by chunks
+  // flush into the slab by FP64 atomic adds that return nothing, so the
+  // workgroup does not wait for a read of the slab: into a workgroup-private
+  // slab (a bin is owned by one thread, so the adds to an address arrive in
+  // program order and the sums are reproducible) or a slab shared by chunks
 #pragma unroll
   for (int i = 0; i < CH; ++i) {
     if (bidx[i] < 0) continue;
     const int64_t o = 5 + 4 * (int64_t)bidx[i];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (ATOMIC) unsafeAtomicAdd(slab + o + c, (double)acc[i][c]);
-      else kahan(slab + o + c, slabc + o + c, (double)acc[i][c]);
-    }
+    for (int c = 0; c < 4; ++c) unsafeAtomicAdd(slab + o + c, (double)acc[i][c]);
   }
 }
 
 // All θ-ranges of the peel-off: full register chunks of Shape<R>::kChunk bins,
 // then the remainder in passes of 4, 2 and 1.
-template <typename R, int GEO, bool MIE, bool ATOMIC, bool CUT>
+template <typename R, int GEO, bool MIE, bool CUT>
 MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc, double* slab,
-                        double* slabc, int E, int tid, int lane, int wave) {
+                        int E, int tid, int lane, int wave) {
   constexpr int CH = Shape<R>::kChunk;
   for (int rg = 0; rg < p.nranges; ++rg) {
     const int jb = p.range_b[rg], je = p.range_e[rg];
@@ -513,20 +515,20 @@ MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F, const R* rtrec, R* e
     const int slot2 = p.range_kind[rg] == 1 ? 0 : 1;
     int c0 = 0;
     for (; c0 + CH <= per; c0 += CH)
-      rt_pass<CH, R, GEO, MIE, ATOMIC, CUT>(p, F, rtrec, esc, slab, slabc, E, tid, lane, wave, jb,
+      rt_pass<CH, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb,
                                             nrow, nb, c0, flag, slot2);
     if (CH > 4 && c0 + 4 <= per) {
-      rt_pass<4, R, GEO, MIE, ATOMIC, CUT>(p, F, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow,
+      rt_pass<4, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow,
                                            nb, c0, flag, slot2);
       c0 += 4;
     }
     if (c0 + 2 <= per) {
-      rt_pass<2, R, GEO, MIE, ATOMIC, CUT>(p, F, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow,
+      rt_pass<2, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow,
                                            nb, c0, flag, slot2);
       c0 += 2;
     }
     if (c0 < per)
-      rt_pass<1, R, GEO, MIE, ATOMIC, CUT>(p, F, rtrec, esc, slab, slabc, E, tid, lane, wave, jb, nrow,
+      rt_pass<1, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow,
                                            nb, c0, flag, slot2);
   }
 }
@@ -540,7 +542,7 @@ MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F, const R* rtrec, R* e
 // every group with atomics.
 template <typename R, int GEO, bool MIE, bool ATOMIC>
 MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* cbrec,
-                      const double* cbdx, double* slab, double* slabc, int E, int tid) {
+                      const double* cbdx, double* slab, int E, int tid) {
   const int nb = p.Ntb * p.Npb;
   // split only iterations with enough events to share (uniform over the workgroup)
   const int G = E >= 8 * p.cb_groups ? p.cb_groups : 1;
@@ -616,10 +618,7 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
     }
     const int64_t ob = 5 + p.nrt + (ATOMIC ? 0 : (int64_t)grp * p.ncb) + 4 * (int64_t)lb;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (ATOMIC) unsafeAtomicAdd(slab + ob + c, (double)acc[c]);
-      else kahan(slab + ob + c, slabc + ob + c, (double)acc[c]);
-    }
+    for (int c = 0; c < 4; ++c) unsafeAtomicAdd(slab + ob + c, (double)acc[c]);
   }
 }
 
@@ -666,12 +665,12 @@ MCRT_DEV void stage_tables(const CoopParams<R>& p, const Lds<R>& L, int tid) {
 }
 
 // Rays [rb, re) of the solve p through one workgroup, with the tables staged
-// in L.  Bins are flushed after every iteration into the slab: a Kahan slab
-// private to the workgroup (sums in slab[0, nout), compensations in
-// slab[nout, 2·nout)), or with ATOMIC a plain FP64 slab shared with other
-// chunks (atomic adds).  The energy budget goes to slab[0..4] at the end and
-// the work counters to p.counters.  slots: this workgroup's SLOTS ray-state
-// records (empty again on return).
+// in L.  Bins are flushed after every iteration into the slab by FP64 atomic
+// adds: sums private to the workgroup in slab[0, nout) (slab[nout, 2·nout)
+// holds compensations, written only for the energy budget and zero
+// elsewhere), or with ATOMIC a slab shared with other chunks.  The energy
+// budget goes to slab[0..4] at the end and the work counters to p.counters.
+// slots: this workgroup's SLOTS ray-state records (empty again on return).
 template <typename R, int GEO, bool MIE, bool CB, bool ATOMIC>
 MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int64_t re,
                        double* slab, Slot<R>* slots) {
@@ -1003,12 +1002,12 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
 
     // ------------------------------- peel-off: bins × events (cpu.jl:6-31)
     if (GEO == 1 && !p.rt_cut)
-      rt_ranges<R, GEO, MIE, ATOMIC, false>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave);
+      rt_ranges<R, GEO, MIE, false>(p, ldsF, rtrec, esc, slab, E, tid, lane, wave);
     else
-      rt_ranges<R, GEO, MIE, ATOMIC, true>(p, ldsF, rtrec, esc, slab, slabc, E, tid, lane, wave);
+      rt_ranges<R, GEO, MIE, true>(p, ldsF, rtrec, esc, slab, E, tid, lane, wave);
 
     // ------------------------------------------ CB bins × events (cpu.jl:38-160)
-    if (CB && p.compute_cb) cb_pass<R, GEO, MIE, ATOMIC>(p, ldsF, Sbins, cbrec, cbdx, slab, slabc, E, tid);
+    if (CB && p.compute_cb) cb_pass<R, GEO, MIE, ATOMIC>(p, ldsF, Sbins, cbrec, cbdx, slab, E, tid);
     __syncthreads();
 
     // ---------------------- weight update and termination (cpu.jl:263-272)
@@ -1723,10 +1722,34 @@ hipError_t coop_launch(CoopPlan& cp, hipStream_t s) {
 // underestimate the variance.
 constexpr int kBatchSlabs = 128;
 
-int64_t default_chunk_rays(int64_t N) {
-  int64_t c = (N + 63) / 64;
-  c = std::max<int64_t>(512, std::min<int64_t>(8192, c));
-  if (N >= 2 && c >= N) c = (N + 1) / 2;  // at least two chunks: a standard error
+// Work of one ray in bin evaluations: (RT bins + 3·CB bins + 1) per event,
+// with events per ray from a rough model of the medium (optical thickness τ
+// of the slab, or radius of the sphere; at most the absorption-limited count
+// ln(1/I_min)/ln(1/ω) and maximum_scattering_times).
+double solve_work_per_ray(const mcrt_config* cfg, const mcrt_geometry* g) {
+  const bool plane = g->kind == MCRT_GEOMETRY_PLANE;
+  const double tau = plane ? g->layers[0].tau : g->tau;
+  const double om = plane ? g->layers[0].omega : g->omega;
+  double ev = 1.0 + 2.0 * tau + 0.5 * tau * tau;
+  if (om < 1.0 && om > 0.0 && cfg->minimum_intensity > 0.0)
+    ev = std::min(ev, 1.0 + std::log(1.0 / cfg->minimum_intensity) / -std::log(om));
+  ev = std::min(ev, (double)std::max<int64_t>(1, cfg->maximum_scattering_times));
+  const double bins = (double)cfg->n_theta * (double)cfg->n_phi +
+                      (cfg->compute_cb ? 3.0 * (double)cfg->n_theta_b * (double)cfg->n_phi_b : 0.0) +
+                      1.0;
+  return ev * bins;
+}
+
+// A launch ends no sooner than its longest chunk, and one workgroup runs a
+// chunk, so the default chunk holds about kChunkWork bin evaluations (some
+// 30 ms of one CU in the mixed form): 64 to 8192 rays, and at least two
+// chunks (a standard error).
+constexpr double kChunkWork = 4e7;
+
+int64_t default_chunk_rays(int64_t N, double work_per_ray) {
+  const double w = std::ceil(kChunkWork / std::max(work_per_ray, 1.0));
+  int64_t c = (int64_t)std::max(64.0, std::min(8192.0, w));
+  if (N >= 2 && c >= N) c = (N + 1) / 2;
   return std::max<int64_t>(1, c);
 }
 
@@ -1754,7 +1777,8 @@ int solve_create_impl(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, cons
   sd.nrt = P.nrt;
   sd.ncb = P.ncb;
   sd.lds_bytes = lds;
-  sd.chunk_rays = chunk_rays > 0 ? chunk_rays : default_chunk_rays(P.N);
+  const double work = solve_work_per_ray(cfg, g);
+  sd.chunk_rays = chunk_rays > 0 ? chunk_rays : default_chunk_rays(P.N, work);
   sd.nchunks = (P.N + sd.chunk_rays - 1) / sd.chunk_rays;
   sd.nbatch = (int)std::min<int64_t>(kBatchSlabs, sd.nchunks);
   std::vector<int64_t> br(sd.nbatch, 0);
@@ -1776,11 +1800,8 @@ int solve_create_impl(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, cons
     return MCRT_ERR_HIP;
   }
   sd.params_dev = pd;
-  // relative work of one chunk: rays × bin evaluations per event × events per ray
-  // (a rough optical-depth model; only the order of chunks depends on it)
-  const double tau = g->kind == MCRT_GEOMETRY_PLANE ? g->layers[0].tau : g->tau;
-  const double ev = 1.0 + 2.0 * tau + 0.5 * tau * tau;
-  sd.cost = (double)sd.chunk_rays * (double)(P.nrt / 4 + 3 * (P.ncb / 4) + 1) * ev;
+  // relative work of one chunk (only the order of chunks depends on it)
+  sd.cost = (double)sd.chunk_rays * work;
   return MCRT_OK;
 }
 }  // namespace
@@ -1796,9 +1817,10 @@ int solve_create(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, const mcr
                                            chunk_rays, alloc, err, cache);
 }
 
-size_t solve_bytes_estimate(const mcrt_config* cfg, bool mixed, int64_t chunk_rays) {
+size_t solve_bytes_estimate(const mcrt_config* cfg, const mcrt_geometry* g, bool mixed,
+                            int64_t chunk_rays) {
   const int64_t N = cfg->number_of_rays;
-  const int64_t cr = chunk_rays > 0 ? chunk_rays : default_chunk_rays(N);
+  const int64_t cr = chunk_rays > 0 ? chunk_rays : default_chunk_rays(N, solve_work_per_ray(cfg, g));
   const int64_t nb = std::min<int64_t>(kBatchSlabs, (N + cr - 1) / cr);
   const int64_t nbins = cfg->n_theta * cfg->n_phi + (cfg->compute_cb ? cfg->n_theta_b * cfg->n_phi_b : 0);
   const int64_t nout = 5 + 4 * nbins;

@@ -550,7 +550,7 @@ extern "C" int mcrt_submit(mcrt_server* S, const mcrt_request* q, uint64_t* tick
   auto* t = new Ticket();
   own(*q, t->r);
   t->mixed = q->precision == MCRT_PRECISION_MIXED;
-  t->need = mcrt::solve_bytes_estimate(q->cfg, t->mixed, q->chunk_rays);
+  t->need = mcrt::solve_bytes_estimate(q->cfg, q->geom, t->mixed, q->chunk_rays);
   t->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> g(S->mu);
@@ -624,6 +624,8 @@ extern "C" int mcrt_result(mcrt_server* S, uint64_t ticket, double A[5], double*
       stats->nonfinite = c[mcrt::kCntNonfinite];
       stats->kernel_ms = std::chrono::duration<double, std::milli>(t->t_done - t->t_start).count();
       stats->total_ms = std::chrono::duration<double, std::milli>(t->t_done - t->t_submit).count();
+      stats->batches = (uint64_t)sd.nbatch;
+      stats->chunks = (uint64_t)sd.nchunks;
     }
   } else if (t->state == MCRT_TICKET_CANCELLED) {
     rc = fail(nullptr, MCRT_ERR_CANCELLED, "ticket %llu was cancelled", (unsigned long long)ticket);

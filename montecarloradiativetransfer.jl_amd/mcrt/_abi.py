@@ -119,6 +119,8 @@ class mcrt_stats(C.Structure):
         ("nonfinite", C.c_uint64),
         ("kernel_ms", C.c_double),
         ("total_ms", C.c_double),
+        ("batches", C.c_uint64),
+        ("chunks", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -226,23 +226,28 @@ def main(argv=None) -> int:
                            keep=a.check, log=log)
     kept = rep.pop("kept")
     # per-request agreement with FP64 on the same stream (batched FP64 re-run)
-    from .stats import batch_count, compare_results
+    from .stats import compare_results
     reqs64 = [Request(r.cfg, r.g, r.I0, seed=r.seed, precision="fp64", chunk_rays=r.chunk_rays)
               for r, _ in kept]
-    checks = []
+    checks, nbs = [], []
     if reqs64:
         with Batch(reqs64) as b:
             b.launch()
             b.wait()
             for i, (r, out) in enumerate(kept):
-                sg = {}
-                ref = b.read(i, sigma=sg)
-                nb = batch_count(r.cfg.number_of_rays, r.chunk_rays)
-                checks.append(compare_results(out, ref, sg, same_stream=True, batches=nb))
+                sg, st = {}, {}
+                ref = b.read(i, sigma=sg, stats=st)
+                nbs.append(int(st["batches"]))
+                checks.append(compare_results(out, ref, sg, same_stream=True, batches=nbs[-1]))
+    # χ²/dof is a test only where the σ has 5 or more degrees of freedom; below
+    # that the |z| > 3 count (Student t) decides
+    chi = [c["chi2dof"] for c, nb in zip(checks, nbs) if nb - 1 >= 5]
     rep["fp64_check"] = {"requests": len(checks),
                          "all_within_3sigma": bool(all(c["ok"] for c in checks)),
                          "worst_exceed3_frac": max((c["exceed3_frac"] for c in checks), default=0.0),
-                         "max_chi2dof": max((c["chi2dof"] for c in checks), default=0.0)}
+                         "batches": nbs,
+                         "chi2_tested": len(chi),
+                         "max_chi2dof_tested": max(chi, default=None)}
     print(json.dumps(rep), flush=True)
     return 0
 

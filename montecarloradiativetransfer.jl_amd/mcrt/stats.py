@@ -60,24 +60,10 @@ def flat(res_tuple):
     return np.concatenate([A, Irt, Icb])
 
 
-def default_chunk_rays(N: int) -> int:
-    """The library's chunking rule for batched and served solves (mcrt_coop.hip,
-    default_chunk_rays): ⌈N/64⌉ clamped to [512, 8192], at least two chunks."""
-    c = max(512, min(8192, -(-N // 64)))
-    if N >= 2 and c >= N:
-        c = (N + 1) // 2
-    return max(1, c)
-
-
-def batch_count(N: int, chunk_rays: int = 0) -> int:
-    """Batches behind the σ of a batched or served solve (at most 128)."""
-    c = chunk_rays if chunk_rays > 0 else default_chunk_rays(N)
-    return min(128, -(-N // c))
-
-
-def compare_results(a, b, sigma_b, *, same_stream=True, batches=32, label="", verbose=False):
+def compare_results(a, b, sigma_b, *, batches, same_stream=True, label="", verbose=False):
     """Agreement of two simulate 7-tuples: `a` against `b` with b's standard
-    errors (a dict with "A", "Irt", "Icb") estimated from `batches` batch means.
+    errors (a dict with "A", "Irt", "Icb") estimated from `batches` batch means
+    (the "batches" entry of the solve's stats).
     With same_stream the two used the same Philox stream and only b's σ counts
     (√2σ otherwise)."""
     s = flat((*sigma_b["A"], sigma_b["Irt"], sigma_b["Icb"]))

@@ -130,3 +130,50 @@ def test_launch_shapes_and_autotuner():
     print(f"[autotune] {res['shape']} {res['rays_per_s']:.4g} rays/s", flush=True)
     assert len(res["trials"]) == 8 and res["shape"] in [(t["slots"], t["workgroups_per_cu"])
                                                          for t in res["trials"]]
+
+
+def _expected_chunks(N, tau, omega, bins, cb_bins, min_int=1e-6, maxscat=10000):
+    """The default chunking rule as include/mcrt.h states it: about 4e7 bin
+    evaluations per chunk, 64..8192 rays, at least two chunks."""
+    import math
+    ev = 1.0 + 2.0 * tau + 0.5 * tau * tau
+    if 0.0 < omega < 1.0:
+        ev = min(ev, 1.0 + math.log(1.0 / min_int) / -math.log(omega))
+    ev = min(ev, maxscat)
+    c = int(max(64.0, min(8192.0, math.ceil(4e7 / (ev * (bins + 3 * cb_bins + 1))))))
+    if N >= 2 and c >= N:
+        c = (N + 1) // 2
+    return -(-N // c)
+
+
+def test_default_chunks_follow_the_work_model():
+    from mcrt.batch import BATCH_THETAB_DEG
+    from mcrt.config import Config, midpoint_polar_grid
+    from mcrt.geometries import SphereGeometry
+    from mcrt.scatterers import RayleighScatterer
+    thin = W.plane_rayleigh_tau1(4096)          # τ = 1, RT only: large chunks
+    tiny = W.plane_rayleigh_tau1(100)           # fewer rays than one chunk: two chunks
+    g = SphereGeometry(R=20.0, scatterer=RayleighScatterer(1.0, 1.0))
+    ct, st_, w = midpoint_polar_grid(g, 180, 48)
+    cfg = Config(g, number_of_rays=1000, maximum_scattering_times=10000, compute_cb=True,
+                 Ntheta=180, Nphi=48, costheta=ct, sintheta=st_, inorm=w,
+                 thetab=np.deg2rad(BATCH_THETAB_DEG), Nphib=8, minimum_intensity=1e-6, tau0=50.0)
+    reqs = [Request(thin[0], thin[1], I0, seed=thin[2], precision="mixed"),
+            Request(tiny[0], tiny[1], I0, seed=tiny[2], precision="mixed"),
+            Request(cfg, g, I0, seed=3, precision="mixed")]
+    want = [_expected_chunks(4096, 1.0, 1.0, 180 * 48, 0),
+            _expected_chunks(100, 1.0, 1.0, 180 * 48, 0),
+            _expected_chunks(1000, 20.0, 1.0, 180 * 48, len(BATCH_THETAB_DEG) * 8)]
+    assert want == [4, 2, 16]
+    with Batch(reqs) as b:
+        assert b.info()["chunks"] == sum(want)
+        b.launch()
+        b.wait()
+        for i, r in enumerate(reqs):
+            st = {}
+            out = b.read(i, stats=st)
+            print(f"[chunks/{i}] chunks={st['chunks']} batches={st['batches']} rays={st['rays']}",
+                  flush=True)
+            assert st["chunks"] == want[i] and st["batches"] == min(128, want[i])
+            assert st["rays"] == r.cfg.number_of_rays and st["nonfinite"] == 0
+            assert abs(sum(out[:5]) - 1.0) < 1e-5

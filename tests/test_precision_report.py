@@ -75,12 +75,3 @@ def test_saved_ranges_merge_by_ray_weight(tmp_path):
     import pytest
     with pytest.raises(ValueError):
         precision.load_merged([pa, pb])
-
-
-def test_chunking_rule_matches_the_library_description():
-    from mcrt.stats import batch_count, default_chunk_rays
-    assert default_chunk_rays(1000) == 512 and batch_count(1000) == 2
-    assert default_chunk_rays(300) == 150 and batch_count(300) == 2
-    assert default_chunk_rays(65536) == 1024 and batch_count(65536) == 64
-    assert default_chunk_rays(10**8) == 8192 and batch_count(10**8) == 128
-    assert batch_count(1) == 1
