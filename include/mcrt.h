@@ -164,7 +164,24 @@ int mcrt_device_count(int* count);
  * Replaces CUDA.jl's implicit device context used by simulate
  * (src/MonteCarloRadiativeTransfer.jl:36-56). */
 int mcrt_context_create(int device, mcrt_ctx** out);
+/* Destroy plans and batches of a context before the context. */
 void mcrt_context_destroy(mcrt_ctx* ctx);
+
+/* Device-resident scatterer tables, shared by content: a plan or batch on the
+ * context whose Mie scatterer has the same Nθ, random_points and tables as
+ * one the context already holds uses those device tables instead of uploading
+ * its own.  mcrt_table_upload uploads (or finds) the FP32 and FP64 tables of
+ * a scatterer and pins them under *handle until mcrt_table_release; unpinned
+ * tables stay cached, least recently used dropped first beyond 1 GiB.  A
+ * Rayleigh scatterer has no tables (a valid handle that pins nothing).
+ * Replaces run_rt's GPU preparation, which moves the scatterer tables to the
+ * device (adapt to CuArray) once before its simulate calls
+ * (src/MonteCarloRadiativeTransfer.jl:97-104 sphere, :194-199 plane). */
+int mcrt_table_upload(mcrt_ctx* ctx, const mcrt_scatterer* sca, uint64_t* handle);
+int mcrt_table_release(mcrt_ctx* ctx, uint64_t handle);
+/* Device bytes held by the context's table cache and its lookup hits and
+ * misses (plans, batches and uploads).  No reference counterpart. */
+int mcrt_table_cache_stats(mcrt_ctx* ctx, int64_t* bytes, uint64_t* hits, uint64_t* misses);
 
 /* ---- host table builders (FP64, no device use) ---------------------------- */
 
@@ -263,10 +280,12 @@ typedef struct mcrt_request {
   uint64_t ray_begin;   /* rays ray_begin .. ray_begin+N-1 of the stream */
   uint32_t instance;
   int32_t precision;    /* MCRT_PRECISION_FP64 or MCRT_PRECISION_MIXED */
-  int64_t chunk_rays;   /* rays per work chunk; 0: the library's default for
-                           this solve (about 4e7 bin evaluations per chunk from
-                           the solve's bins and a model of its events per ray,
-                           64..8192 rays, at least two chunks when N >= 2) */
+  int64_t chunk_rays;   /* rays per work chunk; 0: the default, at least two
+                           chunks when N >= 2.  Batches: ceil(N/64) rays within
+                           512..8192 (throughput).  The server: about 4e7 bin
+                           evaluations per chunk from the solve's bins and a
+                           model of its events per ray, 64..8192 rays (a launch
+                           ends no sooner than its longest chunk: latency). */
 } mcrt_request;
 
 typedef struct mcrt_batch mcrt_batch;
@@ -289,7 +308,8 @@ int mcrt_batch_read(mcrt_batch* batch, int32_t i, double A[5], double* Irt, doub
                     double A_sigma[5], double* Irt_sigma, double* Icb_sigma, mcrt_stats* stats);
 /* Device time (ms) of the last launch: clear, transport and reduction. */
 int mcrt_batch_kernel_ms(mcrt_batch* batch, double* ms);
-/* Chunks in the queue and device bytes held by the batch. */
+/* Chunks in the queue and device bytes held by the batch (scatterer tables
+ * shared through the context's table cache are not counted). */
 int mcrt_batch_info(mcrt_batch* batch, int64_t* chunks, int64_t* device_bytes);
 void mcrt_batch_destroy(mcrt_batch* batch);
 

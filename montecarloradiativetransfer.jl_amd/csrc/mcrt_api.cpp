@@ -20,6 +20,11 @@ namespace {
 
 thread_local std::string g_thread_err;
 constexpr double kPi = 3.141592653589793;
+// Device bytes of scatterer tables a context keeps for later solves once no
+// plan or handle uses them (least recently used go first beyond this).
+constexpr size_t kContextTableBytes = size_t(1) << 30;
+
+int validate_scatterer(mcrt_ctx* ctx, const mcrt_scatterer* s);
 
 // A device allocation that frees itself.
 struct DevBuf {
@@ -73,6 +78,7 @@ struct mcrt_plan {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     for (DevBuf* b : bufs) delete b;
+    if (ctx) mcrt::table_cache_release(ctx->tables, coop.table_refs);
   }
   // Allocate n bytes of device memory owned by the plan.
   void* alloc(size_t n, hipError_t* e) {
@@ -145,6 +151,7 @@ extern "C" int mcrt_context_create(int device, mcrt_ctx** out) {
   }
   c->cu_count = prop.multiProcessorCount;
   c->lds_per_block = prop.sharedMemPerBlock;  // 160 KiB on gfx950
+  c->tables = mcrt::table_cache_create(kContextTableBytes);
   *out = c;
   return MCRT_OK;
 }
@@ -156,7 +163,48 @@ extern "C" void mcrt_context_destroy(mcrt_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
   }
+  for (auto& kv : ctx->table_handles) mcrt::table_cache_release(ctx->tables, kv.second);
+  mcrt::table_cache_destroy(ctx->tables);
   delete ctx;
+}
+
+extern "C" int mcrt_table_upload(mcrt_ctx* ctx, const mcrt_scatterer* sca, uint64_t* handle) {
+  if (!ctx || !sca || !handle) return fail(ctx, MCRT_ERR_INVALID, "table_upload: NULL argument");
+  *handle = 0;
+  int rc = validate_scatterer(ctx, sca);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::vector<const void*> refs;
+  std::string err;
+  rc = mcrt::table_pin(ctx->tables, sca, &refs, &err);
+  if (rc) {
+    mcrt::table_cache_release(ctx->tables, refs);
+    return fail(ctx, rc, "table_upload: %s", err.c_str());
+  }
+  std::lock_guard<std::mutex> g(ctx->table_mu);
+  *handle = ctx->next_table++;
+  ctx->table_handles[*handle] = std::move(refs);
+  return MCRT_OK;
+}
+
+extern "C" int mcrt_table_release(mcrt_ctx* ctx, uint64_t handle) {
+  if (!ctx) return fail(nullptr, MCRT_ERR_INVALID, "table_release: context is NULL");
+  std::lock_guard<std::mutex> g(ctx->table_mu);
+  auto it = ctx->table_handles.find(handle);
+  if (it == ctx->table_handles.end())
+    return fail(ctx, MCRT_ERR_INVALID, "table_release: unknown handle %llu", (unsigned long long)handle);
+  (void)hipSetDevice(ctx->device);
+  mcrt::table_cache_release(ctx->tables, it->second);
+  ctx->table_handles.erase(it);
+  return MCRT_OK;
+}
+
+extern "C" int mcrt_table_cache_stats(mcrt_ctx* ctx, int64_t* bytes, uint64_t* hits, uint64_t* misses) {
+  if (!ctx) return fail(nullptr, MCRT_ERR_INVALID, "table_cache_stats: context is NULL");
+  size_t b = 0;
+  mcrt::table_cache_stats(ctx->tables, &b, hits, misses);
+  if (bytes) *bytes = (int64_t)b;
+  return MCRT_OK;
 }
 
 namespace {
@@ -417,7 +465,7 @@ extern "C" int mcrt_plan_create_shaped(mcrt_ctx* ctx, const mcrt_config* cfg,
     rc = mcrt::coop_create(P->coop, ctx->stream, ctx->cu_count, ctx->lds_per_block, cfg, geom, I0, seed,
                            ray_begin, instance, precision == MCRT_PRECISION_MIXED, shape,
                            [P](size_t n, hipError_t* err) { return P->alloc(n, err); },
-                           P->dA, P->dIrt, P->dIcb, P->dcnt, &ctx->err);
+                           P->dA, P->dIrt, P->dIcb, P->dcnt, &ctx->err, ctx->tables);
     if (rc) g_thread_err = ctx->err;
   }
   if (rc == MCRT_OK) {

@@ -44,6 +44,7 @@ struct mcrt_batch {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     for (void* p : allocs) (void)hipFree(p);
+    for (const mcrt::SolveDev& sd : solves) mcrt::table_cache_release(ctx->tables, sd.table_refs);
   }
   void* alloc(size_t n, hipError_t* e) {
     void* p = nullptr;
@@ -85,7 +86,8 @@ extern "C" int mcrt_batch_create(mcrt_ctx* ctx, int32_t n, const mcrt_request* r
     const mcrt_request& q = reqs[i];
     int rc = mcrt::solve_create(B->solves[i], ctx->lds_per_block, q.cfg, q.geom, q.I0, q.seed,
                                 q.ray_begin, q.instance, q.precision == MCRT_PRECISION_MIXED,
-                                q.chunk_rays, alloc, &ctx->err);
+                                q.chunk_rays, mcrt::ChunkRule::Throughput, alloc, &ctx->err,
+                                ctx->tables);
     if (rc) {
       std::string m = ctx->err;
       delete B;

@@ -34,6 +34,7 @@ struct CoopPlan {
   int geo_kind = 0;
   bool mie = false;
   bool cb = false;
+  std::vector<const void*> table_refs;  // cached tables in use (TableCache)
 };
 
 using DevAlloc = std::function<void*(size_t, hipError_t*)>;
@@ -76,20 +77,22 @@ struct SolveDev {
   std::vector<const void*> table_refs;     // cached tables in use (TableCache)
 };
 
-// Rays per chunk when the caller does not choose: about a fixed number of bin
-// evaluations per chunk from solve_work_per_ray, between 64 and 8192 rays, at
-// least two chunks when N ≥ 2.  A function of the solve alone, so that a
-// solve is cut the same way alone and in any batch.
+// Rays per chunk when the caller does not choose: Throughput (batches) cuts
+// by ray count, ⌈N/64⌉ within 512..8192; Latency (the server) keeps a chunk
+// near a fixed number of bin evaluations from solve_work_per_ray, 64..8192
+// rays.  At least two chunks when N ≥ 2.  A function of the solve alone, so
+// that a solve is cut the same way alone and with any other solves.
+enum class ChunkRule { Throughput, Latency };
 double solve_work_per_ray(const mcrt_config* cfg, const mcrt_geometry* g);
-int64_t default_chunk_rays(int64_t N, double work_per_ray);
+int64_t default_chunk_rays(int64_t N, double work_per_ray, ChunkRule rule);
 // Tables go to memory from alloc, or are shared through cache when given.
 int solve_create(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, const mcrt_geometry* geom,
                  const double I0[4], uint64_t seed, uint64_t ray0, uint32_t instance, bool mixed,
-                 int64_t chunk_rays, const DevAlloc& alloc, std::string* err,
+                 int64_t chunk_rays, ChunkRule rule, const DevAlloc& alloc, std::string* err,
                  TableCache* cache = nullptr);
 // Device bytes of a solve apart from cached tables (for admission).
 size_t solve_bytes_estimate(const mcrt_config* cfg, const mcrt_geometry* g, bool mixed,
-                            int64_t chunk_rays);
+                            int64_t chunk_rays, ChunkRule rule);
 void solve_chunk(const SolveDev& sd, int64_t k, BatchChunk* out);
 // Zero the batch slabs and counters (kernels, so the sequence can be captured).
 hipError_t solve_prepare(const SolveDev& sd, hipStream_t stream);
@@ -108,7 +111,11 @@ int coop_create(CoopPlan& cp, hipStream_t stream, int cu_count, size_t lds_max, 
                 const mcrt_geometry* geom, const double I0[4], uint64_t seed, uint64_t ray0,
                 uint32_t instance, bool mixed, const mcrt_launch_shape& shape,
                 const DevAlloc& alloc, double* dA, double* dIrt, double* dIcb,
-                unsigned long long* dcnt, std::string* err);
+                unsigned long long* dcnt, std::string* err, TableCache* cache = nullptr);
+// Pin a scatterer's FP32 and FP64 device tables in cache (references appended
+// to refs; release them with table_cache_release).  Rayleigh has no tables.
+int table_pin(TableCache* cache, const mcrt_scatterer* s, std::vector<const void*>* refs,
+              std::string* err);
 
 // Clear the workgroup slabs (before the timed transport launch).
 hipError_t coop_prepare(CoopPlan& cp, hipStream_t stream);

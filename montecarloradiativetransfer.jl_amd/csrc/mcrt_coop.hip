@@ -540,6 +540,10 @@ MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F, const R* rtrec, R* e
 // the workgroup (808 bins would leave 216 of 512 threads with half the work).
 // A private slab holds one copy of the CB bins per group; shared slabs take
 // every group with atomics.
+// CB record: [0..7] the scaled forward and reversed fields (n > 1) or I and
+// [32..33] J (n = 1); [8..15] frame matrix of the event direction, [16..23]
+// of -k₂; [24..26] x, [27..29] x₁; [30], [31] the layer τ at x and x₁ (plane)
+// or τ² - x·x and τ² - x₁·x₁ (sphere); [39] kind (0 none, 1 n = 1, 2 n > 1).
 template <typename R, int GEO, bool MIE, bool ATOMIC>
 MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* cbrec,
                       const double* cbdx, double* slab, int E, int tid) {
@@ -547,7 +551,6 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
   // split only iterations with enough events to share (uniform over the workgroup)
   const int G = E >= 8 * p.cb_groups ? p.cb_groups : 1;
   const R tau0 = p.tau0;
-  const R tau2 = p.tau[0] * p.tau[0];
   for (int it = tid; it < nb * G; it += kThreads) {
     const int lb = it % nb, grp = it / nb;
     R b[kBinW];
@@ -567,7 +570,7 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
       if (GEO == 0) ta = kfz > R(0) ? -xa.z * ikz : (xa.z + cr[30]) * -ikz;
       else {
         R kx = kfx * xa.x + kfy * xa.y + kfz * xa.z;
-        ta = sqrt_pos(kx * kx + tau2 - dot(xa, xa)) - kx;
+        ta = sqrt_pos(kx * kx + cr[30]) - kx;
       }
       if (!(ta <= tau0)) continue;
       const Plane<R> qa = scat_plane(cr + 8, kfx, kfy, kfz, b[3], b[4], b[5], b[6], b[7]);
@@ -595,7 +598,7 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
         if (GEO == 0) tb = kfz > R(0) ? -xb.z * ikz : (xb.z + cr[31]) * -ikz;
         else {
           R kx = kfx * xb.x + kfy * xb.y + kfz * xb.z;
-          tb = sqrt_pos(kx * kx + tau2 - dot(xb, xb)) - kx;
+          tb = sqrt_pos(kx * kx + cr[31]) - kx;
         }
         const bool bok = M<R>::fabs(tb + tab) <= tau0;
         const R nb_ = bok ? M<R>::exp(R(-0.5) * (tb + tab)) * qb.inv : R(0);
@@ -972,8 +975,14 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
             for (int c = 0; c < 8; ++c) { cr[8 + c] = ma_[c]; cr[16 + c] = mb_[c]; }
             cr[24] = s.x.x; cr[25] = s.x.y; cr[26] = s.x.z;
             cr[27] = (R)s.x1d.x; cr[28] = (R)s.x1d.y; cr[29] = (R)s.x1d.z;
-            cr[30] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
-            cr[31] = p.isinf[s.idx1] ? R(INFINITY) : p.tau[s.idx1];
+            if (GEO == 0) {
+              cr[30] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
+              cr[31] = p.isinf[s.idx1] ? R(INFINITY) : p.tau[s.idx1];
+            } else {  // τ² - x·x of the sphere's distance (sphere.jl:69-73)
+              const V3<R> xb = {cr[27], cr[28], cr[29]};
+              cr[30] = p.tau[0] * p.tau[0] - dot(s.x, s.x);
+              cr[31] = p.tau[0] * p.tau[0] - dot(xb, xb);
+            }
             double* dx = cbdx + kCbDx * e_me;
             dx[0] = xa.x - s.x1d.x; dx[1] = xa.y - s.x1d.y; dx[2] = xa.z - s.x1d.z;
             dx[3] = ta - tb;  // τab
@@ -986,7 +995,8 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
             cr[0] = s.I[0]; cr[1] = s.I[1]; cr[2] = s.I[2]; cr[3] = s.I[3];
             cr[32] = R(2) * s.I[1]; cr[33] = R(-2) * s.I[2];
             cr[24] = s.x.x; cr[25] = s.x.y; cr[26] = s.x.z;
-            cr[30] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
+            cr[30] = GEO == 0 ? (p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx])
+                              : p.tau[0] * p.tau[0] - dot(s.x, s.x);
             cr[39] = R(1);
           } else {
             cr[39] = R(0);
@@ -1610,10 +1620,11 @@ template <typename R>
 int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* cfg,
                 const mcrt_geometry* g, const double I0[4], uint64_t seed, uint64_t ray0,
                 uint32_t instance, const mcrt_launch_shape& shape, const DevAlloc& alloc,
-                unsigned long long* dcnt, std::string* err) {
+                unsigned long long* dcnt, std::string* err, TableCache* cache) {
   CoopParams<R> P;
   size_t lds = 0;
-  int rc = init_params<R>(P, &lds, lds_max, cfg, g, I0, seed, ray0, instance, alloc, dcnt, err);
+  int rc = init_params<R>(P, &lds, lds_max, cfg, g, I0, seed, ray0, instance, alloc, dcnt, err,
+                          cache, &cp.table_refs);
   if (rc) return rc;
   const bool mie = P.sca.kind != 0;
   const bool cb = P.compute_cb != 0;
@@ -1697,15 +1708,28 @@ int coop_create(CoopPlan& cp, hipStream_t, int cu_count, size_t lds_max, const m
                 const mcrt_geometry* geom, const double I0[4], uint64_t seed, uint64_t ray0,
                 uint32_t instance, bool mixed, const mcrt_launch_shape& shape,
                 const DevAlloc& alloc, double* dA, double* dIrt, double* dIcb,
-                unsigned long long* dcnt, std::string* err) {
+                unsigned long long* dcnt, std::string* err, TableCache* cache) {
   cp.mixed = mixed;
   cp.dA = dA; cp.dIrt = dIrt; cp.dIcb = dIcb;
   if (cu_count <= 0) cu_count = 256;
   if (lds_max == 0) lds_max = 160 * 1024;
   return mixed ? create_impl<float>(cp, cu_count, lds_max, cfg, geom, I0, seed, ray0, instance, shape,
-                                    alloc, dcnt, err)
+                                    alloc, dcnt, err, cache)
                : create_impl<double>(cp, cu_count, lds_max, cfg, geom, I0, seed, ray0, instance, shape,
-                                     alloc, dcnt, err);
+                                     alloc, dcnt, err, cache);
+}
+
+int table_pin(TableCache* cache, const mcrt_scatterer* s, std::vector<const void*>* refs,
+              std::string* err) {
+  if (s->kind != MCRT_SCATTERER_MIE) return MCRT_OK;
+  const DevAlloc none = [](size_t, hipError_t* e) -> void* { *e = hipErrorInvalidValue; return nullptr; };
+  int rc = MCRT_OK;
+  ScaView<float> vf;
+  fill_sca<float>(vf, s, none, &rc, err, cache, refs);
+  if (rc) return rc;
+  ScaView<double> vd;
+  fill_sca<double>(vd, s, none, &rc, err, cache, refs);
+  return rc;
 }
 
 hipError_t coop_prepare(CoopPlan& cp, hipStream_t s) {
@@ -1740,15 +1764,26 @@ double solve_work_per_ray(const mcrt_config* cfg, const mcrt_geometry* g) {
   return ev * bins;
 }
 
-// A launch ends no sooner than its longest chunk, and one workgroup runs a
-// chunk, so the default chunk holds about kChunkWork bin evaluations (some
-// 30 ms of one CU in the mixed form): 64 to 8192 rays, and at least two
-// chunks (a standard error).
+// One workgroup runs a chunk, and the rays of a chunk fill its slots only
+// until the chunk's last rays are in flight: the tail of a chunk, where a few
+// long rays keep a workgroup iterating with most slots empty, costs ~25-40 µs
+// per iteration whatever the chunk size.
+//  * Throughput (batches): ⌈N/64⌉ rays, 512 to 8192, so that tails are a small
+//    part of a chunk and a solve still spreads over many workgroups.
+//  * Latency (the server): a launch ends no sooner than its longest chunk, so
+//    a chunk holds about kChunkWork bin evaluations (some 30 ms of one CU in
+//    the mixed form), 64 to 8192 rays.
+// Both give at least two chunks when N ≥ 2 (a standard error).
 constexpr double kChunkWork = 4e7;
 
-int64_t default_chunk_rays(int64_t N, double work_per_ray) {
-  const double w = std::ceil(kChunkWork / std::max(work_per_ray, 1.0));
-  int64_t c = (int64_t)std::max(64.0, std::min(8192.0, w));
+int64_t default_chunk_rays(int64_t N, double work_per_ray, ChunkRule rule) {
+  int64_t c;
+  if (rule == ChunkRule::Latency) {
+    const double w = std::ceil(kChunkWork / std::max(work_per_ray, 1.0));
+    c = (int64_t)std::max(64.0, std::min(8192.0, w));
+  } else {
+    c = std::max<int64_t>(512, std::min<int64_t>(8192, (N + 63) / 64));
+  }
   if (N >= 2 && c >= N) c = (N + 1) / 2;
   return std::max<int64_t>(1, c);
 }
@@ -1757,7 +1792,7 @@ namespace {
 template <typename R>
 int solve_create_impl(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, const mcrt_geometry* g,
                       const double I0[4], uint64_t seed, uint64_t ray0, uint32_t instance,
-                      int64_t chunk_rays, const DevAlloc& alloc, std::string* err,
+                      int64_t chunk_rays, ChunkRule rule, const DevAlloc& alloc, std::string* err,
                       TableCache* cache) {
   hipError_t e;
   sd.counters = (unsigned long long*)alloc(sizeof(unsigned long long) * kCntN, &e);
@@ -1778,7 +1813,7 @@ int solve_create_impl(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, cons
   sd.ncb = P.ncb;
   sd.lds_bytes = lds;
   const double work = solve_work_per_ray(cfg, g);
-  sd.chunk_rays = chunk_rays > 0 ? chunk_rays : default_chunk_rays(P.N, work);
+  sd.chunk_rays = chunk_rays > 0 ? chunk_rays : default_chunk_rays(P.N, work, rule);
   sd.nchunks = (P.N + sd.chunk_rays - 1) / sd.chunk_rays;
   sd.nbatch = (int)std::min<int64_t>(kBatchSlabs, sd.nchunks);
   std::vector<int64_t> br(sd.nbatch, 0);
@@ -1808,19 +1843,21 @@ int solve_create_impl(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, cons
 
 int solve_create(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, const mcrt_geometry* geom,
                  const double I0[4], uint64_t seed, uint64_t ray0, uint32_t instance, bool mixed,
-                 int64_t chunk_rays, const DevAlloc& alloc, std::string* err, TableCache* cache) {
+                 int64_t chunk_rays, ChunkRule rule, const DevAlloc& alloc, std::string* err,
+                 TableCache* cache) {
   if (lds_max == 0) lds_max = 160 * 1024;
   lds_max -= 16;  // the batch kernel's static LDS (the chunk index)
   return mixed ? solve_create_impl<float>(sd, lds_max, cfg, geom, I0, seed, ray0, instance,
-                                          chunk_rays, alloc, err, cache)
+                                          chunk_rays, rule, alloc, err, cache)
                : solve_create_impl<double>(sd, lds_max, cfg, geom, I0, seed, ray0, instance,
-                                           chunk_rays, alloc, err, cache);
+                                           chunk_rays, rule, alloc, err, cache);
 }
 
 size_t solve_bytes_estimate(const mcrt_config* cfg, const mcrt_geometry* g, bool mixed,
-                            int64_t chunk_rays) {
+                            int64_t chunk_rays, ChunkRule rule) {
   const int64_t N = cfg->number_of_rays;
-  const int64_t cr = chunk_rays > 0 ? chunk_rays : default_chunk_rays(N, solve_work_per_ray(cfg, g));
+  const int64_t cr =
+      chunk_rays > 0 ? chunk_rays : default_chunk_rays(N, solve_work_per_ray(cfg, g), rule);
   const int64_t nb = std::min<int64_t>(kBatchSlabs, (N + cr - 1) / cr);
   const int64_t nbins = cfg->n_theta * cfg->n_phi + (cfg->compute_cb ? cfg->n_theta_b * cfg->n_phi_b : 0);
   const int64_t nout = 5 + 4 * nbins;

@@ -3,17 +3,30 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/mcrt.h"
 
-// A device context: one HIP stream on one device.
+namespace mcrt {
+struct TableCache;
+}
+
+// A device context: one HIP stream on one device, and the device tables of
+// the scatterers its solves used (shared by content; handles pin them).
 struct mcrt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
   int cu_count = 0;
   size_t lds_per_block = 0;
+  mcrt::TableCache* tables = nullptr;
+  std::mutex table_mu;
+  std::map<uint64_t, std::vector<const void*>> table_handles;  // handle -> pinned entries
+  uint64_t next_table = 1;
 };
 
 namespace mcrt {

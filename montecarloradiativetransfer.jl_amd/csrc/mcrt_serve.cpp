@@ -258,7 +258,8 @@ void mcrt_server::admit(std::vector<Ticket*>& batch) {
     std::string err;
     const mcrt_request& q = t->r.req;
     int rc = mcrt::solve_create(t->sd, ctx->lds_per_block, q.cfg, q.geom, q.I0, q.seed, q.ray_begin,
-                                q.instance, t->mixed, q.chunk_rays, alloc, &err, cache);
+                                q.instance, t->mixed, q.chunk_rays, mcrt::ChunkRule::Latency,
+                                alloc, &err, cache);
     hipError_t e = hipSuccess;
     // the clear runs on the set-up stream; launches wait for setup_ev
     if (!rc) e = mcrt::solve_prepare(t->sd, setup);
@@ -550,7 +551,8 @@ extern "C" int mcrt_submit(mcrt_server* S, const mcrt_request* q, uint64_t* tick
   auto* t = new Ticket();
   own(*q, t->r);
   t->mixed = q->precision == MCRT_PRECISION_MIXED;
-  t->need = mcrt::solve_bytes_estimate(q->cfg, q->geom, t->mixed, q->chunk_rays);
+  t->need = mcrt::solve_bytes_estimate(q->cfg, q->geom, t->mixed, q->chunk_rays,
+                                       mcrt::ChunkRule::Latency);
   t->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> g(S->mu);

@@ -4,7 +4,8 @@ PlaneGeometry, SphereGeometry, RayleighScatterer, MieScatterer, simulate,
 run_rt).  The transport runs in libmcrt_hip.so; this package marshals the
 reference's objects into its C ABI (include/mcrt.h).
 """
-from .api import Context, Plan, get_context, mueller_plane, mueller_sphere, run_rt, simulate
+from .api import (Context, Plan, ResidentTables, get_context, mueller_plane, mueller_sphere, run_rt,
+                  simulate, table_cache_stats)
 from .config import DEFAULT_THEB, Config, midpoint_polar_grid
 from .geometries import (ArgumentError, Layer, PlaneGeometry, SphereGeometry, gausslegendre,
                          initial_weights)
@@ -17,4 +18,5 @@ __all__ = [
     "MieScatterer", "simulate", "run_rt", "Plan", "Context", "get_context", "bhmie",
     "calculate_csrn", "gausslegendre", "initial_weights", "midpoint_polar_grid", "mean_free_path",
     "single_scattering_albedo", "ArgumentError", "mueller_plane", "mueller_sphere",
+    "ResidentTables", "table_cache_stats",
 ]

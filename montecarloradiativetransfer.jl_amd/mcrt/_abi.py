@@ -169,6 +169,10 @@ SIGNATURES = {
     "mcrt_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "mcrt_context_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "mcrt_context_destroy": (None, [_vp]),
+    "mcrt_table_upload": (C.c_int, [_vp, C.POINTER(mcrt_scatterer), C.POINTER(C.c_uint64)]),
+    "mcrt_table_release": (C.c_int, [_vp, C.c_uint64]),
+    "mcrt_table_cache_stats": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint64)]),
     "mcrt_bhmie": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32,
                              _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mcrt_calculate_csrn": (C.c_int, [_dp, C.c_int32, C.c_int32, _dp]),

@@ -256,6 +256,49 @@ def _frame(columns: dict):
         return columns
 
 
+class ResidentTables:
+    """A scatterer's tables pinned in device memory (mcrt_table_upload): solves
+    on the device whose scatterer has the same tables use this copy.  Release
+    with release() or a with-block."""
+
+    def __init__(self, scatterer, device: int = 0):
+        from .scatterers import to_struct as sca_struct
+        self.ctx = get_context(device)
+        keep: list = []
+        st = sca_struct(scatterer, keep)
+        h = C.c_uint64()
+        _abi.check(self.ctx.lib.mcrt_table_upload(self.ctx.handle, C.byref(st), C.byref(h)),
+                   self.ctx.handle)
+        self.handle = h.value
+
+    def release(self) -> None:
+        if self.handle:
+            _abi.check(self.ctx.lib.mcrt_table_release(self.ctx.handle, C.c_uint64(self.handle)),
+                       self.ctx.handle)
+            self.handle = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.release()
+
+
+def table_cache_stats(device: int = 0) -> dict:
+    """Bytes, hits and misses of the device context's scatterer-table cache."""
+    ctx = get_context(device)
+    b, h, m = C.c_int64(), C.c_uint64(), C.c_uint64()
+    _abi.check(ctx.lib.mcrt_table_cache_stats(ctx.handle, C.byref(b), C.byref(h), C.byref(m)),
+               ctx.handle)
+    return {"bytes": b.value, "hits": h.value, "misses": m.value}
+
+
+def _scatterers(g: AbstractGeometry) -> list:
+    if isinstance(g, SphereGeometry):
+        return [g.scatterer]
+    return [layer.scatterer for layer in g.layers]
+
+
 def run_rt(cfg: Config, g: AbstractGeometry, *, use_gpu: bool = True, seed: int = DEFAULT_SEED,
            precision: str = "fp64", device: int = 0, verbose: bool = False):
     """run_rt(cfg, g; use_gpu) — MonteCarloRadiativeTransfer.jl:87-291.
@@ -266,12 +309,19 @@ def run_rt(cfg: Config, g: AbstractGeometry, *, use_gpu: bool = True, seed: int 
     g.validate()
     IS = g.initial_I()
     runs = []
-    for i in range(IS.shape[1]):
-        I = IS[:, i].copy()
-        if verbose:
-            print(f"I = {I}")
-        runs.append((I, simulate(cfg, g, I, use_gpu=use_gpu, seed=seed + i, precision=precision,
-                                 device=device)))
+    # the tables go to the device once for all columns (the reference's adapt
+    # before its simulate loop, :97-104 and :194-199)
+    pins = [ResidentTables(s, device) for s in _scatterers(g)] if use_gpu else []
+    try:
+        for i in range(IS.shape[1]):
+            I = IS[:, i].copy()
+            if verbose:
+                print(f"I = {I}")
+            runs.append((I, simulate(cfg, g, I, use_gpu=use_gpu, seed=seed + i,
+                                     precision=precision, device=device)))
+    finally:
+        for t in pins:
+            t.release()
     deg = 180.0 / math.pi
     if isinstance(g, SphereGeometry):
         MRT, MCB, G = mueller_sphere(cfg, runs)

@@ -132,28 +132,32 @@ def test_launch_shapes_and_autotuner():
                                                          for t in res["trials"]]
 
 
-def _expected_chunks(N, tau, omega, bins, cb_bins, min_int=1e-6, maxscat=10000):
-    """The default chunking rule as include/mcrt.h states it: about 4e7 bin
-    evaluations per chunk, 64..8192 rays, at least two chunks."""
+def _expected_chunks(N, tau, omega, bins, cb_bins, rule, min_int=1e-6, maxscat=10000):
+    """The default chunking rules as include/mcrt.h states them: batches cut by
+    ray count (ceil(N/64) within 512..8192), the server by work (about 4e7 bin
+    evaluations per chunk, 64..8192 rays); at least two chunks."""
     import math
-    ev = 1.0 + 2.0 * tau + 0.5 * tau * tau
-    if 0.0 < omega < 1.0:
-        ev = min(ev, 1.0 + math.log(1.0 / min_int) / -math.log(omega))
-    ev = min(ev, maxscat)
-    c = int(max(64.0, min(8192.0, math.ceil(4e7 / (ev * (bins + 3 * cb_bins + 1))))))
+    if rule == "latency":
+        ev = 1.0 + 2.0 * tau + 0.5 * tau * tau
+        if 0.0 < omega < 1.0:
+            ev = min(ev, 1.0 + math.log(1.0 / min_int) / -math.log(omega))
+        ev = min(ev, maxscat)
+        c = int(max(64.0, min(8192.0, math.ceil(4e7 / (ev * (bins + 3 * cb_bins + 1))))))
+    else:
+        c = max(512, min(8192, -(-N // 64)))
     if N >= 2 and c >= N:
         c = (N + 1) // 2
     return -(-N // c)
 
 
-def test_default_chunks_follow_the_work_model():
+def _chunk_cases():
     from mcrt.batch import BATCH_THETAB_DEG
     from mcrt.config import Config, midpoint_polar_grid
     from mcrt.geometries import SphereGeometry
     from mcrt.scatterers import RayleighScatterer
-    thin = W.plane_rayleigh_tau1(4096)          # τ = 1, RT only: large chunks
+    thin = W.plane_rayleigh_tau1(4096)          # τ = 1, RT only
     tiny = W.plane_rayleigh_tau1(100)           # fewer rays than one chunk: two chunks
-    g = SphereGeometry(R=20.0, scatterer=RayleighScatterer(1.0, 1.0))
+    g = SphereGeometry(R=20.0, scatterer=RayleighScatterer(1.0, 1.0))   # thick, CB on
     ct, st_, w = midpoint_polar_grid(g, 180, 48)
     cfg = Config(g, number_of_rays=1000, maximum_scattering_times=10000, compute_cb=True,
                  Ntheta=180, Nphi=48, costheta=ct, sintheta=st_, inorm=w,
@@ -161,19 +165,34 @@ def test_default_chunks_follow_the_work_model():
     reqs = [Request(thin[0], thin[1], I0, seed=thin[2], precision="mixed"),
             Request(tiny[0], tiny[1], I0, seed=tiny[2], precision="mixed"),
             Request(cfg, g, I0, seed=3, precision="mixed")]
-    want = [_expected_chunks(4096, 1.0, 1.0, 180 * 48, 0),
-            _expected_chunks(100, 1.0, 1.0, 180 * 48, 0),
-            _expected_chunks(1000, 20.0, 1.0, 180 * 48, len(BATCH_THETAB_DEG) * 8)]
-    assert want == [4, 2, 16]
+    args = [(4096, 1.0, 1.0, 180 * 48, 0), (100, 1.0, 1.0, 180 * 48, 0),
+            (1000, 20.0, 1.0, 180 * 48, len(BATCH_THETAB_DEG) * 8)]
+    return reqs, args
+
+
+def test_default_chunks_of_batches_and_served_solves():
+    from mcrt.serve import Server
+    reqs, args = _chunk_cases()
+    want_b = [_expected_chunks(*a, rule="throughput") for a in args]
+    want_s = [_expected_chunks(*a, rule="latency") for a in args]
+    assert want_b == [8, 2, 2] and want_s == [4, 2, 16]
     with Batch(reqs) as b:
-        assert b.info()["chunks"] == sum(want)
+        assert b.info()["chunks"] == sum(want_b)
         b.launch()
         b.wait()
         for i, r in enumerate(reqs):
             st = {}
             out = b.read(i, stats=st)
-            print(f"[chunks/{i}] chunks={st['chunks']} batches={st['batches']} rays={st['rays']}",
-                  flush=True)
-            assert st["chunks"] == want[i] and st["batches"] == min(128, want[i])
+            print(f"[chunks/batch/{i}] chunks={st['chunks']} batches={st['batches']}", flush=True)
+            assert st["chunks"] == want_b[i] and st["batches"] == min(128, want_b[i])
             assert st["rays"] == r.cfg.number_of_rays and st["nonfinite"] == 0
+            assert abs(sum(out[:5]) - 1.0) < 1e-5
+    with Server() as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        for i, t in enumerate(tickets):
+            st = {}
+            out = srv.result(t, stats=st)
+            print(f"[chunks/server/{i}] chunks={st['chunks']} batches={st['batches']}", flush=True)
+            assert st["chunks"] == want_s[i] and st["batches"] == min(128, want_s[i])
+            assert st["rays"] == reqs[i].cfg.number_of_rays and st["nonfinite"] == 0
             assert abs(sum(out[:5]) - 1.0) < 1e-5

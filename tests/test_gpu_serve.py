@@ -36,12 +36,17 @@ def test_result_is_the_same_alone_and_in_the_mix():
     err = float(np.max(np.abs(outs[1] - alone)))
     print(f"[serve/alone-vs-mix] max|Δ|={err:.3e}", flush=True)
     assert err <= 1e-12 * float(np.max(np.abs(alone)))
-    # and the same as the batched launch of that request
-    with Batch([reqs[1]]) as b:
+    # and, cut into the same chunks, the same as the batched launch of that
+    # request (batches and the server have different default chunk sizes)
+    r = reqs[1]
+    fixed = Request(r.cfg, r.g, r.I0, seed=r.seed, precision=r.precision, chunk_rays=512)
+    with Server() as srv:
+        served = flat(srv.result(srv.submit(fixed)))
+    with Batch([fixed]) as b:
         b.launch()
         b.wait()
         ref = flat(b.read(0))
-    assert float(np.max(np.abs(ref - alone))) <= 1e-12 * float(np.max(np.abs(ref)))
+    assert float(np.max(np.abs(ref - served))) <= 1e-12 * float(np.max(np.abs(ref)))
     for r, o in zip(reqs, outs):
         assert np.all(np.isfinite(o)) and abs(o[:5].sum() - 1.0) < 1e-5
     assert st["completed"] == 5 and st["failed"] == 0
