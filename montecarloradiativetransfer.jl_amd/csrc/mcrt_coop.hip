@@ -551,15 +551,25 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
   // split only iterations with enough events to share (uniform over the workgroup)
   const int G = E >= 8 * p.cb_groups ? p.cb_groups : 1;
   const R tau0 = p.tau0;
-  for (int it = tid; it < nb * G; it += kThreads) {
-    const int lb = it % nb, grp = it / nb;
-    R b[kBinW];
+  const int total = nb * G;
+  // a work item's bin constants; the next item's are loaded before the flush
+  // of the current one (loads issued after its atomics would wait for them)
+  struct Item { R b[kBinW]; double kik[3]; };
+  auto load = [&](int it, Item& q) {
+    const int lb = it % nb;
     const R* src = p.cb_bin + kBinStride * lb;
 #pragma unroll
-    for (int c = 0; c < kBinW; ++c) b[c] = src[c];
-    const R kfx = b[0], kfy = b[1], kfz = b[2], ikz = b[10];
+    for (int c = 0; c < kBinW; ++c) q.b[c] = src[c];
     const double* kk = p.cb_kik + 4 * lb;
-    const double kikx = kk[0], kiky = kk[1], kikz = kk[2];
+    q.kik[0] = kk[0]; q.kik[1] = kk[1]; q.kik[2] = kk[2];
+  };
+  Item cur;
+  if (tid < total) load(tid, cur);
+  for (int it = tid; it < total;) {
+    const int lb = it % nb, grp = it / nb;
+    const R* b = cur.b;
+    const R kfx = b[0], kfy = b[1], kfz = b[2], ikz = b[10];
+    const double kikx = cur.kik[0], kiky = cur.kik[1], kikz = cur.kik[2];
     R acc[4] = {R(0), R(0), R(0), R(0)};
     for (int e = grp; e < E; e += G) {
       const R* cr = cbrec + kCbRec * e;
@@ -619,9 +629,15 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[c] += o[c];
     }
+    const int nit = it + kThreads;
+    Item nxt;
+    if (nit < total) load(nit, nxt);
     const int64_t ob = 5 + p.nrt + (ATOMIC ? 0 : (int64_t)grp * p.ncb) + 4 * (int64_t)lb;
 #pragma unroll
     for (int c = 0; c < 4; ++c) unsafeAtomicAdd(slab + ob + c, (double)acc[c]);
+    if (nit >= total) break;
+    it = nit;
+    cur = nxt;
   }
 }
 
