@@ -58,11 +58,16 @@ template <> struct Shape<float> {
   static constexpr int kSlots = 256;
   static constexpr int kChunk = 4;
   static constexpr bool kStageS = true;  // S table in LDS
+  // ρ² below which a bin counts as aligned with the event direction (the
+  // polar branch): FP32 rounding leaves ρ² ≈ 1e-13 for exactly (anti)parallel
+  // directions, e.g. the θb = 0 CB bins at oblique incidence (ρ < 1e-5 rad)
+  static constexpr float kPolar2 = 1e-10f;
 };
 template <> struct Shape<double> {
   static constexpr int kSlots = 128;
   static constexpr int kChunk = 2;
   static constexpr bool kStageS = false;
+  static constexpr double kPolar2 = 1e-24;  // the reference's x² + y² < 1e-24
 };
 
 template <typename R> struct CoopParams {
@@ -269,9 +274,9 @@ template <typename R> MCRT_DEV void frame_matrix(Fr<R> f, R m[8]) {
 // so that every rotation of a peel-off is a ratio over the one factor
 // inv = 1/ρ² and needs no square root.  The polar branch of
 // spherical_from_cartesian (utils.jl:12-14, x² + y² < 1e-24: ϕₖ = 0,
-// θₖ ∈ {0, π}) is taken by a wave-uniform test, since exact alignment of a
-// bin with an event direction is rare (a divergent branch that is skipped
-// when no lane of the wave takes it).
+// θₖ ∈ {0, π}; here ρ² < Shape<R>::kPolar2) is taken by a wave-uniform test,
+// since exact alignment of a bin with an event direction is rare (a
+// divergent branch that is skipped when no lane of the wave takes it).
 template <typename R> struct Plane { R mu, rx, ry, pn, w, inv; };
 
 // The general case; returns ρ² (q.inv is not set).
@@ -285,7 +290,7 @@ MCRT_DEV R plane_raw(Plane<R>& q, const R* m, R kx, R ky, R kz, R e1, R e2, R g0
   return q.rx * q.rx + q.ry * q.ry;
 }
 
-// The polar case (ρ² < 1e-24): Ehₖ = -ϕ̂, Evₖ = sign(μ)·θ̂; returns ρ² := 1.
+// The polar case (ρ² < Shape<R>::kPolar2): Ehₖ = -ϕ̂, Evₖ = sign(μ)·θ̂; returns ρ² := 1.
 template <typename R>
 MCRT_DEV R plane_polar(Plane<R>& q, const R* m, R e1, R e2) {
   const R u = e1 * m[0] + e2 * m[3], v = e1 * m[1] + e2 * m[4];
@@ -300,7 +305,7 @@ template <typename R>
 MCRT_DEV Plane<R> scat_plane(const R* m, R kx, R ky, R kz, R e1, R e2, R g0, R g1, R g2) {
   Plane<R> q;
   R r2 = plane_raw(q, m, kx, ky, kz, e1, e2, g0, g1, g2);
-  if (r2 < R(1e-24)) r2 = plane_polar(q, m, e1, e2);
+  if (r2 < Shape<R>::kPolar2) r2 = plane_polar(q, m, e1, e2);
   q.inv = M<R>::div(R(1), r2);
   return q;
 }
@@ -454,12 +459,12 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
     for (int i = 0; i < CH; ++i) {
       r2[i] = plane_raw(qs[i], rec, bk[i][0], bk[i][1], bk[i][2], bk[i][3], bk[i][4], bk[i][5],
                         bk[i][6], bk[i][7]);
-      polar |= r2[i] < R(1e-24);
+      polar |= r2[i] < Shape<R>::kPolar2;
     }
     if (polar) {
 #pragma unroll
       for (int i = 0; i < CH; ++i)
-        if (r2[i] < R(1e-24)) r2[i] = plane_polar(qs[i], rec, bk[i][3], bk[i][4]);
+        if (r2[i] < Shape<R>::kPolar2) r2[i] = plane_polar(qs[i], rec, bk[i][3], bk[i][4]);
     }
     R part = R(0);
 #pragma unroll

@@ -1,0 +1,98 @@
+"""GPU parity on the configurations the main parity cases leave out: sphere with
+Rayleigh scatterers, oblique incidence on a Mie slab, two layers, a
+semi-infinite layer, an active τ₀ cut-off, paths cut short by
+maximum_scattering_times / minimum_intensity, the reference's default
+Gauss-Legendre polar grid, and circularly polarised input.  Every form runs
+the same Philox stream as the oracle, so the criteria are those of
+test_gpu_parity (binomial |z| > 3 count and χ²/dof, exact energy telescoping,
+event counts within 2%).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import mcrt
+import oracle
+from mcrt.config import Config
+from mcrt.geometries import Layer, PlaneGeometry, SphereGeometry
+from mcrt.scatterers import MieScatterer, RayleighScatterer
+from statcheck import compare, flat, oracle_flat
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ["fp64_plain", "fp64", "mixed"]
+
+
+def _mie():
+    return MieScatterer.create(0.3, complex(1.45, 0.01), 0.55, 0.1, Ntheta=181, random_points=500)
+
+
+def _case(name):
+    if name == "sphere_rayleigh_cb":
+        g = SphereGeometry(R=2.0, scatterer=RayleighScatterer(1.0, 1.0))
+        return Config(g, number_of_rays=1024, Ntheta=90, Nphi=16, compute_cb=True), g, 21, (1.0, -1.0, 0.0, 0.0)
+    if name == "plane_mie_oblique_cb":
+        s = _mie()
+        g = PlaneGeometry(math.radians(130.0), math.radians(40.0),
+                          [Layer(d=0.8 * s.kl, scatterer=s, tau=0.8, omega=s.omega)])
+        return Config(g, number_of_rays=1024, Ntheta=60, Nphi=24, compute_cb=True), g, 22, (1.0, 0.0, 1.0, 0.0)
+    if name == "plane_two_layers":
+        s = RayleighScatterer(1.0, 1.0)
+        g = PlaneGeometry(math.pi, 0.0, [Layer(d=0.5, scatterer=s, tau=0.5, omega=1.0),
+                                         Layer(d=1.0, scatterer=s, tau=1.0, omega=0.8)])
+        return Config(g, number_of_rays=1024, Ntheta=60, Nphi=24, compute_cb=False), g, 23, (1.0, -1.0, 0.0, 0.0)
+    if name == "plane_semi_infinite":
+        s = RayleighScatterer(0.8, 1.0)
+        g = PlaneGeometry(math.radians(150.0), 0.0, [Layer(d=-1.0, scatterer=s, tau=1.0, omega=0.8)])
+        return Config(g, number_of_rays=1024, Ntheta=60, Nphi=24, compute_cb=True), g, 24, (1.0, 1.0, 0.0, 0.0)
+    if name == "sphere_mie_tau0_cut":
+        s = _mie()
+        g = SphereGeometry(R=3.0 * s.kl, scatterer=s, tau=3.0, omega=s.omega)
+        return (Config(g, number_of_rays=1024, Ntheta=60, Nphi=16, compute_cb=True, tau0=2.5), g, 25,
+                (1.0, 0.0, 0.0, 1.0))
+    if name == "sphere_short_paths":
+        s = _mie()
+        g = SphereGeometry(R=4.0 * s.kl, scatterer=s, tau=4.0, omega=s.omega)
+        return (Config(g, number_of_rays=1024, Ntheta=60, Nphi=16, compute_cb=True,
+                       maximum_scattering_times=3, minimum_intensity=0.05), g, 26, (1.0, -1.0, 0.0, 0.0))
+    raise KeyError(name)
+
+
+CASES = ["sphere_rayleigh_cb", "plane_mie_oblique_cb", "plane_two_layers", "plane_semi_infinite",
+         "sphere_mie_tau0_cut", "sphere_short_paths"]
+_ORACLE = {}
+
+
+def _oracle(name):
+    if name not in _ORACLE:
+        cfg, g, seed, I0 = _case(name)
+        _ORACLE[name] = oracle.Solve(cfg, g).simulate(list(I0), seed, nbatch=32)
+    return _ORACLE[name]
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("case", CASES)
+def test_edge_case_vs_oracle(case, form):
+    cfg, g, seed, I0 = _case(case)
+    print(f"case={case} form={form} N={cfg.number_of_rays}", flush=True)
+    ref = _oracle(case)
+    st = {}
+    try:
+        out = mcrt.simulate(cfg, g, list(I0), seed=seed, precision=form, stats=st)
+    except mcrt._abi.MCRTError as e:
+        if e.code == mcrt._abi.MCRT_ERR_UNSUPPORTED:  # e.g. layers with different scatterers
+            pytest.skip(str(e))
+        raise
+    a = flat(out)
+    b = oracle_flat(ref)
+    sig = oracle.sigma_of_mean(ref) * np.sqrt(2.0)
+    assert abs(sum(out[:5]) - 1.0) < (1e-5 if form == "mixed" else 1e-9)
+    assert st["rays"] == cfg.number_of_rays and st["nonfinite"] == 0
+    dead_rel = 1e-6 if form == "mixed" else 1e-12
+    r = compare(a, b, sig, floor=1e-15, dead_rel=dead_rel, label=f"{case}/{form}")
+    assert r["ok"], r
+    ev_ref = ref["counters"][0]
+    assert abs(st["events"] - ev_ref) <= 0.02 * ev_ref + 5
+    if case == "sphere_short_paths":
+        assert st["events"] <= 3 * cfg.number_of_rays
