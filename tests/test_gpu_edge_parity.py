@@ -96,3 +96,48 @@ def test_edge_case_vs_oracle(case, form):
     assert abs(st["events"] - ev_ref) <= 0.02 * ev_ref + 5
     if case == "sphere_short_paths":
         assert st["events"] <= 3 * cfg.number_of_rays
+
+
+def _big_bins():
+    s = MieScatterer.create(0.3, complex(1.45, 0.01), 0.55, 0.1, Ntheta=721, random_points=1000)
+    g = SphereGeometry(R=2.0 * s.kl, scatterer=s, tau=2.0, omega=s.omega)
+    thetab = np.deg2rad(np.concatenate([np.arange(101) * 0.05, np.arange(6.0, 181.0)]))
+    cfg = Config(g, number_of_rays=256, Ntheta=360, Nphi=96, compute_cb=True, thetab=thetab,
+                 Nphib=16)
+    return cfg, g, 27, (1.0, -1.0, 0.0, 0.0)
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_large_bin_grids_vs_oracle(form):
+    """34560 RT bins (68 per thread, many register chunks) and 4416 CB bins."""
+    cfg, g, seed, I0 = _big_bins()
+    if "big" not in _ORACLE:
+        _ORACLE["big"] = oracle.Solve(cfg, g).simulate(list(I0), seed, nbatch=32)
+    ref = _ORACLE["big"]
+    st = {}
+    out = mcrt.simulate(cfg, g, list(I0), seed=seed, precision=form, stats=st)
+    assert abs(sum(out[:5]) - 1.0) < (1e-5 if form == "mixed" else 1e-9)
+    assert st["rays"] == cfg.number_of_rays and st["nonfinite"] == 0
+    r = compare(flat(out), oracle_flat(ref), oracle.sigma_of_mean(ref) * np.sqrt(2.0), floor=1e-15,
+                dead_rel=1e-6 if form == "mixed" else 1e-12, label=f"big/{form}")
+    assert r["ok"], r
+
+
+def test_mie_tables_beyond_lds_are_refused_by_the_cooperative_forms():
+    """The cooperative forms stage the Mie tables in LDS: Nθ up to 2301 with CB in the
+    mixed form; a larger table is a clear MCRT_ERR_UNSUPPORTED, and fp64_plain (tables
+    in device memory) still runs it."""
+    ok = MieScatterer.create(0.3, complex(1.45, 0.01), 0.55, 0.1, Ntheta=2301, random_points=1000)
+    big = MieScatterer.create(0.3, complex(1.45, 0.01), 0.55, 0.1, Ntheta=2401, random_points=1000)
+    for s, fits in ((ok, True), (big, False)):
+        g = SphereGeometry(R=1.0 * s.kl, scatterer=s, tau=1.0, omega=s.omega)
+        cfg = Config(g, number_of_rays=512, Ntheta=60, Nphi=16, compute_cb=True)
+        if fits:
+            out = mcrt.simulate(cfg, g, [1.0, -1.0, 0.0, 0.0], seed=3, precision="mixed")
+            assert abs(sum(out[:5]) - 1.0) < 1e-5
+        else:
+            with pytest.raises(mcrt._abi.MCRTError) as e:
+                mcrt.simulate(cfg, g, [1.0, -1.0, 0.0, 0.0], seed=3, precision="mixed")
+            assert e.value.code == mcrt._abi.MCRT_ERR_UNSUPPORTED and "LDS" in str(e.value)
+            out = mcrt.simulate(cfg, g, [1.0, -1.0, 0.0, 0.0], seed=3, precision="fp64_plain")
+            assert abs(sum(out[:5]) - 1.0) < 1e-9
