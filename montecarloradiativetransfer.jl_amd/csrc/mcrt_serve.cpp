@@ -182,6 +182,10 @@ struct Ticket {
   int inflight = 0;           // launches still running with chunks of this solve
   bool finished = false;      // the launch with its reduction and read-back completed
   bool released = false;      // device memory freed
+  // read-back of the results: one pinned block (host_pool) holding the sums,
+  // the standard errors and the counters
+  void* h_block = nullptr;
+  size_t h_bytes = 0;
   double* h_out = nullptr;
   double* h_sig = nullptr;
   unsigned long long* h_cnt = nullptr;
@@ -190,6 +194,11 @@ struct Ticket {
 
 constexpr int kLanes = 2;
 constexpr int kLaneChunks = 1 << 16;  // chunk-list capacity of one launch
+// Admitted solves at a time.  An admitted solve holds device and pinned
+// memory; under overload, admitting everything the budget allows (tens of
+// thousands of small solves) only multiplies live allocations beyond what the
+// next launch windows can use.  The rest wait in the queue.
+constexpr size_t kMaxActive = 1024;
 
 }  // namespace
 
@@ -265,10 +274,13 @@ void mcrt_server::admit(std::vector<Ticket*>& batch) {
     if (!rc) e = mcrt::solve_prepare(t->sd, setup);
     if (!rc && e == hipSuccess) {
       const size_t n = (size_t)t->sd.nout;
-      t->h_out = (double*)host_pool->get(sizeof(double) * n, &e);
-      if (e == hipSuccess) t->h_sig = (double*)host_pool->get(sizeof(double) * n, &e);
-      if (e == hipSuccess)
-        t->h_cnt = (unsigned long long*)host_pool->get(sizeof(unsigned long long) * mcrt::kCntN, &e);
+      t->h_bytes = sizeof(double) * 2 * n + sizeof(unsigned long long) * mcrt::kCntN;
+      t->h_block = host_pool->get(t->h_bytes, &e);
+      if (t->h_block) {
+        t->h_out = (double*)t->h_block;
+        t->h_sig = t->h_out + n;
+        t->h_cnt = (unsigned long long*)(t->h_sig + n);
+      }
     }
     if (!rc && e != hipSuccess) {
       rc = e == hipErrorOutOfMemory ? MCRT_ERR_NOMEM : MCRT_ERR_HIP;
@@ -365,21 +377,21 @@ bool mcrt_server::launch(Lane& L) {
   }
   if (e == hipSuccess) e = hipEventRecord(L.ev, L.s);
   if (e != hipSuccess) {
-    // the lane is unusable: fail its solves once the stream drains
+    // the lane is unusable: its solves launch nothing more, and the sweep
+    // publishes them as failed once no launch holds them (a ticket is never
+    // published while the server still lists it: its owner may free it)
     (void)hipStreamSynchronize(L.s);
     std::lock_guard<std::mutex> g(mu);
     for (Ticket* t : L.members) {
       --t->inflight;
-      t->cancel = true;  // the sweep frees it
-      t->state = MCRT_TICKET_FAILED;
-      t->err_code = MCRT_ERR_HIP;
-      t->err = hipGetErrorString(e);
-      t->t_done = Clock::now();
-      ++st.failed;
+      t->cancel = true;
+      if (!t->err_code) {
+        t->err_code = MCRT_ERR_HIP;
+        t->err = hipGetErrorString(e);
+      }
     }
     L.members.clear();
     L.finishing.clear();
-    cv_done.notify_all();
     return false;
   }
   L.busy = true;
@@ -419,6 +431,7 @@ void mcrt_server::run() {
           continue;
         }
         if (in_use + t->need > budget) break;  // wait for memory to free up
+        if (active.size() + batch.size() >= kMaxActive) break;
         pending.pop_front();
         in_use += t->need;
         t->admitted = true;
@@ -462,8 +475,10 @@ void mcrt_server::run() {
         std::lock_guard<std::mutex> g(mu);
         for (Ticket* t : over) {
           t->t_done = Clock::now();
-          if (t->state != MCRT_TICKET_RUNNING) continue;  // failed on the way
-          if (t->finished) {
+          if (t->err_code) {  // a launch holding its chunks failed
+            t->state = MCRT_TICKET_FAILED;
+            ++st.failed;
+          } else if (t->finished) {
             t->state = MCRT_TICKET_DONE;
             ++st.completed;
             st.rays += (uint64_t)t->sd.N;
@@ -636,10 +651,7 @@ extern "C" int mcrt_result(mcrt_server* S, uint64_t ticket, double A[5], double*
               (unsigned long long)ticket, t->err.c_str());
   }
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(t->t_done - t->t_submit).count();
-  const size_t n = (size_t)t->sd.nout;
-  S->host_pool->put(t->h_out, sizeof(double) * n);
-  S->host_pool->put(t->h_sig, sizeof(double) * n);
-  S->host_pool->put(t->h_cnt, sizeof(unsigned long long) * mcrt::kCntN);
+  S->host_pool->put(t->h_block, t->h_bytes);
   delete t;
   return rc;
 }
@@ -686,10 +698,7 @@ extern "C" void mcrt_server_destroy(mcrt_server* S) {
   for (auto& kv : S->tickets) {
     Ticket* t = kv.second;
     S->release(t);
-    const size_t n = (size_t)t->sd.nout;
-    S->host_pool->put(t->h_out, sizeof(double) * n);
-    S->host_pool->put(t->h_sig, sizeof(double) * n);
-    S->host_pool->put(t->h_cnt, sizeof(unsigned long long) * mcrt::kCntN);
+    S->host_pool->put(t->h_block, t->h_bytes);
     delete t;
   }
   for (auto& L : S->lanes) {
