@@ -341,14 +341,17 @@ bool mcrt_server::launch(Lane& L) {
   size_t lds = 0;
   L.members.clear();
   L.finishing.clear();
-  for (size_t i = 0; i < elig.size(); ++i) {
-    if (!take[i]) continue;
-    Ticket* t = elig[i];
-    t->next_chunk += take[i];
-    ++t->inflight;
-    L.members.push_back(t);
-    if (t->next_chunk >= t->sd.nchunks) L.finishing.push_back(t);
-    lds = std::max(lds, t->sd.lds_bytes);
+  {
+    std::lock_guard<std::mutex> g(mu);  // mcrt_cancel reads next_chunk
+    for (size_t i = 0; i < elig.size(); ++i) {
+      if (!take[i]) continue;
+      Ticket* t = elig[i];
+      t->next_chunk += take[i];
+      ++t->inflight;
+      L.members.push_back(t);
+      if (t->next_chunk >= t->sd.nchunks) L.finishing.push_back(t);
+      lds = std::max(lds, t->sd.lds_bytes);
+    }
   }
   for (size_t j = 0; j < items.size(); ++j) mcrt::solve_chunk(items[j].t->sd, items[j].k, &L.hchunks[j]);
   hipError_t e = hipMemcpyAsync(L.dchunks, L.hchunks, sizeof(mcrt::BatchChunk) * items.size(),
