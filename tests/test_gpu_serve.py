@@ -28,6 +28,8 @@ def test_result_is_the_same_alone_and_in_the_mix():
     reqs = _reqs()
     with Server() as srv:
         t = srv.submit(reqs[1])
+        assert srv.poll(t) in ("queued", "running", "done")
+        assert srv.wait(t) == "done" and srv.poll(t) == "done"
         alone = flat(srv.result(t))
         ts = [srv.submit(r) for r in reqs]  # reqs[1] again, now among others
         outs = [flat(srv.result(x)) for x in ts]
