@@ -92,6 +92,8 @@ def test_edge_case_vs_oracle(case, form):
     dead_rel = 1e-6 if form == "mixed" else 1e-12
     r = compare(a, b, sig, floor=1e-15, dead_rel=dead_rel, label=f"{case}/{form}")
     assert r["ok"], r
+    # same stream: the per-ray histories agree, so every bin agrees far inside σ
+    assert r["max_abs_z"] < (1.0 if form == "mixed" else 0.5), r
     ev_ref = ref["counters"][0]
     assert abs(st["events"] - ev_ref) <= 0.02 * ev_ref + 5
     if case == "sphere_short_paths":
@@ -121,6 +123,7 @@ def test_large_bin_grids_vs_oracle(form):
     r = compare(flat(out), oracle_flat(ref), oracle.sigma_of_mean(ref) * np.sqrt(2.0), floor=1e-15,
                 dead_rel=1e-6 if form == "mixed" else 1e-12, label=f"big/{form}")
     assert r["ok"], r
+    assert r["max_abs_z"] < (1.0 if form == "mixed" else 0.5), r
 
 
 def test_mie_tables_beyond_lds_are_refused_by_the_cooperative_forms():

@@ -63,6 +63,8 @@ def test_form_vs_oracle(case, form):
     dead_rel = 1e-6 if form == "mixed" else 1e-12
     r = compare(a, b, sig, floor=1e-15, dead_rel=dead_rel, label=f"{case}/{form}")
     assert r["ok"], r
+    # same stream: the per-ray histories agree, so every bin agrees far inside σ
+    assert r["max_abs_z"] < (1.0 if form == "mixed" else 0.5), r
     # same stream: event counts agree closely
     ev_ref = ref["counters"][0]
     assert abs(st["events"] - ev_ref) <= 0.02 * ev_ref + 5
@@ -101,6 +103,7 @@ def test_ray_range_matches_oracle(form):
     r = compare(flat(out), oracle_flat(ref), sig, floor=1e-15, dead_rel=dead_rel,
                 label=f"range/{form}")
     assert r["ok"], r
+    assert r["max_abs_z"] < (1.0 if form == "mixed" else 0.5), r
     assert abs(st["events"] - ref["counters"][0]) <= 0.02 * ref["counters"][0] + 5
     # a different stream position really gives different rays
     base = mcrt.simulate(cfg, g, I0, seed=seed, precision=form)
