@@ -171,7 +171,8 @@ template <> MCRT_DEV float theta_index(const CoopParams<float>& p, float mu) {
   float c = p.acs[0];
 #pragma unroll
   for (int k = 1; k < 8; ++k) c = __builtin_fmaf(c, a, p.acs[k]);
-  const float r = __builtin_amdgcn_sqrtf(__builtin_fmaxf(1.0f - a, 0.0f)) * c;
+  // 1 - a clamped to [0, 1] (|μ| can exceed 1 by rounding): the clamp bit of the subtraction
+  const float r = __builtin_amdgcn_sqrtf(__builtin_amdgcn_fmed3f(1.0f - a, 0.0f, 1.0f)) * c;
   return mu < 0.0f ? p.pis - r : r;
 }
 
@@ -277,7 +278,11 @@ template <typename R> MCRT_DEV void frame_matrix(Fr<R> f, R m[8]) {
 // θₖ ∈ {0, π}; here ρ² < Shape<R>::kPolar2) is taken by a wave-uniform test,
 // since exact alignment of a bin with an event direction is rare (a
 // divergent branch that is skipped when no lane of the wave takes it).
-template <typename R> struct Plane { R mu, rx, ry, pn, w, inv; };
+// c2 = rx² - ry² = ρ²·cos2ϕₖ and r2 = ρ² are kept for the Stokes rotation.
+template <typename R> struct Plane { R mu, rx, ry, pn, w, c2, r2, inv; };
+
+MCRT_DEV float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+MCRT_DEV double fma_r(double a, double b, double c) { return ::fma(a, b, c); }
 
 // The general case; returns ρ² (q.inv is not set).
 template <typename R>
@@ -287,7 +292,10 @@ MCRT_DEV R plane_raw(Plane<R>& q, const R* m, R kx, R ky, R kz, R e1, R e2, R g0
   q.mu = m[2] * kx + m[5] * ky + m[7] * kz;
   q.pn = m[2] * g0 + m[5] * g1 + m[7] * g2;
   q.w = m[2] * e1 + m[5] * e2;
-  return q.rx * q.rx + q.ry * q.ry;
+  const R ry2 = q.ry * q.ry;
+  q.c2 = fma_r(q.rx, q.rx, -ry2);
+  q.r2 = fma_r(q.rx, q.rx, ry2);
+  return q.r2;
 }
 
 // The polar case (ρ² < Shape<R>::kPolar2): Ehₖ = -ϕ̂, Evₖ = sign(μ)·θ̂; returns ρ² := 1.
@@ -298,6 +306,8 @@ MCRT_DEV R plane_polar(Plane<R>& q, const R* m, R e1, R e2) {
   q.ry = R(0);
   q.pn = -v;
   q.w = q.mu > R(0) ? -u : (q.mu < R(0) ? u : R(0));
+  q.c2 = R(1);
+  q.r2 = R(1);
   return R(1);
 }
 
@@ -319,7 +329,7 @@ MCRT_DEV void peel_I(const CoopParams<R>& p, const R* F, const R* I, const R* J,
                      const Plane<R>& q, R o[4]) {
   R P[4];
   phase_tab<R, MIE>(p, F, q.mu, P);
-  const R c2 = q.rx * q.rx - q.ry * q.ry;  // ρ²·cos2ϕₖ
+  const R c2 = q.c2;                      // ρ²·cos2ϕₖ
   const R sh = q.rx * q.ry;                // -ρ²·sin2ϕₖ / 2
   const R i1 = (c2 * I[1] + sh * J[1]) * q.inv;
   const R i2 = (sh * J[0] + c2 * I[2]) * q.inv;
@@ -327,8 +337,10 @@ MCRT_DEV void peel_I(const CoopParams<R>& p, const R* F, const R* I, const R* J,
   const R a1 = P[1] * I[0] + P[0] * i1;
   const R a2 = P[2] * i2 + P[3] * I[3];
   const R a3 = -P[3] * i2 + P[2] * I[3];
-  const R c2p = q.pn * q.pn - q.w * q.w;   // ρ²·cos2ψ
-  const R s2p = R(2) * q.w * q.pn;         // ρ²·sin2ψ
+  // ρ²·cos2ψ = pn² - w² = ρ² - 2w² (pn² + w² = ρ²: cos²ψ + sin²ψ = 1), ρ²·sin2ψ = 2w·pn
+  const R w2 = q.w + q.w;
+  const R c2p = fma_r(-w2, q.w, q.r2);
+  const R s2p = w2 * q.pn;
   o[0] = a0;
   o[1] = c2p * a1 + s2p * a2;
   o[2] = -s2p * a1 + c2p * a2;
@@ -422,11 +434,12 @@ template <typename R> MCRT_DEV Slot<R>* slot_block(void* base, int block) {
 // lb = tid + kThreads·(c0 + i) of the θ-range [jb, jb + nrow) × Nϕ; the escape
 // sum of each event is wave-reduced and added by lane 0 to esc in chunk order.
 // RT record: m[0..7], I[8..11], J[12..13], K·x[14..16], [17] K·τ of the layer
-// (plane) or K²·(τ² - x·x) (sphere), [18] flags; K = ExpK<R>::k.
+// (plane) or K²·(τ² - x·x) (sphere), [18] 1 or 0: near the upper (first) boundary,
+// [19] 1 or 0: near the lower (last) boundary; K = ExpK<R>::k.
 template <int CH, typename R, int GEO, bool MIE, bool CUT>
 MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc, double* slab,
                       int E, int tid, int lane, int wave, int jb, int nrow, int nb,
-                      int c0, int flag, int slot2) {
+                      int c0, int fw, int slot2) {
   R bk[CH][kBinW];
   int bidx[CH];
   R acc[CH][4];
@@ -447,8 +460,7 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
   const R tau0k = p.tau0k;
   for (int e = 0; e < E; ++e) {
     const R* rec = rtrec + kRtRec * e;
-    const int fl = (int)rec[18];
-    if (!(fl & flag)) continue;
+    if (rec[fw] == R(0)) continue;  // not near the range's boundary
     // the scattering planes of all CH bins first, with one (rarely taken)
     // branch for the polar case, so that the lookups and Stokes algebra of the
     // CH bins form one straight-line block the scheduler can interleave
@@ -516,25 +528,25 @@ MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F, const R* rtrec, R* e
     const int nrow = je - jb;
     const int nb = nrow * p.Np;  // bins of this range
     const int per = (nb + kThreads - 1) / kThreads;
-    const int flag = p.range_kind[rg] == 1 ? 1 : 2;
+    const int fw = p.range_kind[rg] == 1 ? 18 : 19;  // record word of the near flag
     const int slot2 = p.range_kind[rg] == 1 ? 0 : 1;
     int c0 = 0;
     for (; c0 + CH <= per; c0 += CH)
       rt_pass<CH, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb,
-                                            nrow, nb, c0, flag, slot2);
+                                            nrow, nb, c0, fw, slot2);
     if (CH > 4 && c0 + 4 <= per) {
       rt_pass<4, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow,
-                                           nb, c0, flag, slot2);
+                                           nb, c0, fw, slot2);
       c0 += 4;
     }
     if (c0 + 2 <= per) {
       rt_pass<2, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow,
-                                           nb, c0, flag, slot2);
+                                           nb, c0, fw, slot2);
       c0 += 2;
     }
     if (c0 < per)
       rt_pass<1, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow,
-                                           nb, c0, flag, slot2);
+                                           nb, c0, fw, slot2);
   }
 }
 
@@ -953,7 +965,8 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
           if (GEO == 0) rec[17] = p.isinf[s.idx] ? R(INFINITY) : K * p.tau[s.idx];
           else rec[17] = K * K * (p.tau[0] * p.tau[0] - dot(s.x, s.x));
         }
-        rec[18] = R(near_up | (near_lo << 1));
+        rec[18] = R(near_up);
+        rec[19] = R(near_lo);
         for (int w = 0; w < kWaves; ++w) {
           esc[(e_me * kWaves + w) * 2] = R(0);
           esc[(e_me * kWaves + w) * 2 + 1] = R(0);
@@ -1049,7 +1062,7 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
         Iref += esc[(e_me * kWaves + w) * 2];
         Itrans += esc[(e_me * kWaves + w) * 2 + 1];
       }
-      const int fl = (int)rtrec[kRtRec * e_me + 18];
+      const int fl = (rtrec[kRtRec * e_me + 18] != R(0)) | ((rtrec[kRtRec * e_me + 19] != R(0)) << 1);
       for (int rg = 0; rg < p.nranges; ++rg)
         if (fl & (p.range_kind[rg] == 1 ? 1 : 2))
           cnt[kCntRt] += (unsigned long long)(p.range_e[rg] - p.range_b[rg]) * p.Np;
