@@ -79,7 +79,7 @@ template <typename R> struct CoopParams {
   R tau0;
   R tau0k;      // tau0 · ExpK<R>::k
   R acs[8];     // FP32 arccos coefficients times (Nθ_table - 1)/π
-  R pis;        // π · (Nθ_table - 1)/π
+  R pis;        // π · (Nθ_table - 1)/π = Nθ_table - 1, the last node
   int compute_cb;
   int geo_kind;  // 0 plane, 1 sphere
   int nl;
@@ -181,13 +181,13 @@ MCRT_DEV float sqrt_pos(float x) { return __builtin_amdgcn_sqrtf(__builtin_fmaxf
 MCRT_DEV double sqrt_pos(double x) { return ::sqrt(x < 0.0 ? 0.0 : x); }
 
 // Split sn into the node index i0 ≤ imax and the fraction f = sn - i0.
-MCRT_DEV int node_of(float sn, int imax, float& f) {
-  const float s = __builtin_fminf(sn, (float)imax);
+MCRT_DEV int node_of(float sn, float imax, float& f) {
+  const float s = __builtin_fminf(sn, imax);
   f = __builtin_amdgcn_fractf(s);
   return (int)s;
 }
-MCRT_DEV int node_of(double sn, int imax, double& f) {
-  const double s = sn < (double)imax ? sn : (double)imax;
+MCRT_DEV int node_of(double sn, double imax, double& f) {
+  const double s = sn < imax ? sn : imax;
   const int i = (int)s;
   f = s - (double)i;
   return i;
@@ -206,7 +206,7 @@ MCRT_DEV void phase_tab(const CoopParams<R>& p, const R* F, R mu, R P[4]) {
     P[3] = R(0);
   } else {
     R f;
-    const int i0 = node_of(theta_index(p, mu), p.sca.Nt - 1, f);
+    const int i0 = node_of(theta_index(p, mu), p.pis, f);
     const R* a = F + 4 * i0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) P[c] = a[c] + f * (a[4 + c] - a[c]);
@@ -223,7 +223,7 @@ MCRT_DEV void amp_tab(const CoopParams<R>& p, const R* S, R mu, Cx<R>& S11, Cx<R
     S22 = {mu * R(0), mu};
   } else {
     R f;
-    const int i0 = node_of(theta_index(p, mu), p.sca.Nt - 1, f);
+    const int i0 = node_of(theta_index(p, mu), p.pis, f);
     const R g = R(1) - f;
     const R* a = S + 4 * i0;
     S11 = {a[0] + g * (a[0] - a[4]), a[1] + g * (a[1] - a[5])};
@@ -458,9 +458,14 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
     for (int c = 0; c < 4; ++c) acc[i][c] = R(0);
   }
   const R tau0k = p.tau0k;
+  unsigned long long near = 0;  // near flags of events e & ~63 .. (e & ~63) + 63, one per lane
   for (int e = 0; e < E; ++e) {
+    if ((e & 63) == 0) {
+      const int ee = e + lane;
+      near = __ballot(ee < E && rtrec[kRtRec * ee + fw] != R(0));
+    }
+    if (!((near >> (e & 63)) & 1ull)) continue;  // not near the range's boundary
     const R* rec = rtrec + kRtRec * e;
-    if (rec[fw] == R(0)) continue;  // not near the range's boundary
     // the scattering planes of all CH bins first, with one (rarely taken)
     // branch for the polar case, so that the lookups and Stokes algebra of the
     // CH bins form one straight-line block the scheduler can interleave
@@ -1545,7 +1550,7 @@ int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_co
                                 -0.0501743046, 0.0889789874, -0.2145988016, 1.5707963050};
     P.scale = (R)sc;
     for (int k = 0; k < 8; ++k) P.acs[k] = (R)(c[k] * sc);
-    P.pis = (R)(3.141592653589793 * sc);
+    P.pis = mie ? (R)(s0->n_theta - 1) : R(0);  // π·sc
     P.tau0k = (R)(cfg->tau0 * (double)ExpK<R>::k);
     // inside a sphere every distance to the boundary is at most the diameter
     P.rt_cut = !(g->kind == MCRT_GEOMETRY_SPHERE && 2.0 * g->tau * (1.0 + 1e-6) < cfg->tau0);
