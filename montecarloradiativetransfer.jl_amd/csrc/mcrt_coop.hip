@@ -432,7 +432,8 @@ template <typename R> MCRT_DEV Slot<R>* slot_block(void* base, int block) {
 // One pass of the peel-off (cpu.jl:11-28) over CH register-resident bins per
 // thread and all E events of the iteration.  Thread tid owns the bins
 // lb = tid + kThreads·(c0 + i) of the θ-range [jb, jb + nrow) × Nϕ; the escape
-// sum of each event is wave-reduced and added by lane 0 to esc in chunk order.
+// sum of each event is wave-reduced and added to esc in chunk order by the
+// lane of the event within its block of 64.
 // RT record: m[0..7], I[8..11], J[12..13], K·x[14..16], [17] K·τ of the layer
 // (plane) or K²·(τ² - x·x) (sphere), [18] 1 or 0: near the upper (first) boundary,
 // [19] 1 or 0: near the lower (last) boundary; K = ExpK<R>::k.
@@ -458,56 +459,60 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
     for (int c = 0; c < 4; ++c) acc[i][c] = R(0);
   }
   const R tau0k = p.tau0k;
-  unsigned long long near = 0;  // near flags of events e & ~63 .. (e & ~63) + 63, one per lane
-  for (int e = 0; e < E; ++e) {
-    if ((e & 63) == 0) {
-      const int ee = e + lane;
-      near = __ballot(ee < E && rtrec[kRtRec * ee + fw] != R(0));
-    }
-    if (!((near >> (e & 63)) & 1ull)) continue;  // not near the range's boundary
-    const R* rec = rtrec + kRtRec * e;
-    // the scattering planes of all CH bins first, with one (rarely taken)
-    // branch for the polar case, so that the lookups and Stokes algebra of the
-    // CH bins form one straight-line block the scheduler can interleave
-    Plane<R> qs[CH];
-    R r2[CH];
-    bool polar = false;
+  // events in blocks of 64: lane l holds the near flag and, at the end, the
+  // wave's escape sum of event e0 + l, added to esc by that lane
+  for (int e0 = 0; e0 < E; e0 += 64) {
+    const int el = e0 + lane;
+    const unsigned long long near = __ballot(el < E && rtrec[kRtRec * el + fw] != R(0));
+    R tot = R(0);
+    const int e1 = min(E, e0 + 64);
+    for (int e = e0; e < e1; ++e) {
+      if (!((near >> (e - e0)) & 1ull)) continue;  // not near the range's boundary
+      const R* rec = rtrec + kRtRec * e;
+      // the scattering planes of all CH bins first, with one (rarely taken)
+      // branch for the polar case, so that the lookups and Stokes algebra of the
+      // CH bins form one straight-line block the scheduler can interleave
+      Plane<R> qs[CH];
+      R r2[CH];
+      bool polar = false;
 #pragma unroll
-    for (int i = 0; i < CH; ++i) {
-      r2[i] = plane_raw(qs[i], rec, bk[i][0], bk[i][1], bk[i][2], bk[i][3], bk[i][4], bk[i][5],
-                        bk[i][6], bk[i][7]);
-      polar |= r2[i] < Shape<R>::kPolar2;
-    }
-    if (polar) {
-#pragma unroll
-      for (int i = 0; i < CH; ++i)
-        if (r2[i] < Shape<R>::kPolar2) r2[i] = plane_polar(qs[i], rec, bk[i][3], bk[i][4]);
-    }
-    R part = R(0);
-#pragma unroll
-    for (int i = 0; i < CH; ++i) {
-      Plane<R>& q = qs[i];
-      q.inv = M<R>::div(R(1), r2[i]);
-      R o[4];
-      peel_I<R, MIE>(p, F, rec + 8, rec + 12, q, o);
-      R t;  // K·distance_to_boundary
-      if (GEO == 0) {  // plane.jl:97-105
-        t = (bk[i][2] > R(0) ? rec[16] : rec[16] + rec[17]) * -bk[i][9];
-      } else {         // sphere.jl:69-73
-        const R kx = bk[i][0] * rec[14] + bk[i][1] * rec[15] + bk[i][2] * rec[16];
-        t = sqrt_pos(kx * kx + rec[17]) - kx;
+      for (int i = 0; i < CH; ++i) {
+        r2[i] = plane_raw(qs[i], rec, bk[i][0], bk[i][1], bk[i][2], bk[i][3], bk[i][4], bk[i][5],
+                          bk[i][6], bk[i][7]);
+        polar |= r2[i] < Shape<R>::kPolar2;
       }
-      // the τ₀ cut-off (cpu.jl:20); CUT = false when no distance can exceed it
-      const R w = (!CUT || t <= tau0k) ? bk[i][8] * ExpK<R>::exp_neg(t) : R(0);
-      const R wi = w * q.inv;
-      acc[i][0] += o[0] * w;
-      acc[i][1] += o[1] * wi;
-      acc[i][2] += o[2] * wi;
-      acc[i][3] += o[3] * w;
-      part += o[0] * w;
+      if (polar) {
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+          if (r2[i] < Shape<R>::kPolar2) r2[i] = plane_polar(qs[i], rec, bk[i][3], bk[i][4]);
+      }
+      R part = R(0);
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        Plane<R>& q = qs[i];
+        q.inv = M<R>::div(R(1), r2[i]);
+        R o[4];
+        peel_I<R, MIE>(p, F, rec + 8, rec + 12, q, o);
+        R t;  // K·distance_to_boundary
+        if (GEO == 0) {  // plane.jl:97-105
+          t = (bk[i][2] > R(0) ? rec[16] : rec[16] + rec[17]) * -bk[i][9];
+        } else {         // sphere.jl:69-73
+          const R kx = bk[i][0] * rec[14] + bk[i][1] * rec[15] + bk[i][2] * rec[16];
+          t = sqrt_pos(kx * kx + rec[17]) - kx;
+        }
+        // the τ₀ cut-off (cpu.jl:20); CUT = false when no distance can exceed it
+        const R w = (!CUT || t <= tau0k) ? bk[i][8] * ExpK<R>::exp_neg(t) : R(0);
+        const R wi = w * q.inv;
+        acc[i][0] += o[0] * w;
+        acc[i][1] += o[1] * wi;
+        acc[i][2] += o[2] * wi;
+        acc[i][3] += o[3] * w;
+        part += o[0] * w;
+      }
+      const R sum = wave_sum(part);
+      if (lane == e - e0) tot = sum;
     }
-    part = wave_sum(part);
-    if (lane == 0) esc[(e * kWaves + wave) * 2 + slot2] += part;
+    if (el < E) esc[(el * kWaves + wave) * 2 + slot2] += tot;
   }
   // flush into the slab by FP64 atomic adds that return nothing, so the
   // workgroup does not wait for a read of the slab: into a workgroup-private
