@@ -162,7 +162,10 @@ int mcrt_device_count(int* count);
 
 /* Create a context on a device: one HIP stream, table cache, scratch.
  * Replaces CUDA.jl's implicit device context used by simulate
- * (src/MonteCarloRadiativeTransfer.jl:36-56). */
+ * (src/MonteCarloRadiativeTransfer.jl:36-56).  The plan, batch and table
+ * calls of one context are made from one host thread at a time (they share
+ * the context's stream and its last-error message); a server's calls may come
+ * from any thread (their messages are per calling thread). */
 int mcrt_context_create(int device, mcrt_ctx** out);
 /* Destroy plans and batches of a context before the context. */
 void mcrt_context_destroy(mcrt_ctx* ctx);
