@@ -5,7 +5,7 @@
  * The library replaces the GPU branch of the reference's `simulate`
  * (src/MonteCarloRadiativeTransfer.jl:36-56, which launches `gpu_kernel`,
  * src/gpu.jl:177-296) and the scatterer table builders it depends on
- * (src/scatterers/mie.jl:43-51,136-227; src/utils.jl:263-287;
+ * (src/scatterers/mie.jl:36-44,129-216; src/utils.jl:263-287;
  * FastGaussQuadrature.gausslegendre used at src/geometries/plane.jl:132 and
  * src/geometries/sphere.jl:86).  Every entry point is `extern "C"`, takes plain
  * pointers and sizes, and never throws: it returns an integer status, and
@@ -188,7 +188,7 @@ int mcrt_table_cache_stats(mcrt_ctx* ctx, int64_t* bytes, uint64_t* hits, uint64
 
 /* ---- host table builders (FP64, no device use) ---------------------------- */
 
-/* bhmie(x, m; Nθ, nextra, custom_nstop) — src/scatterers/mie.jl:136-227.
+/* bhmie(x, m; Nθ, nextra, custom_nstop) — src/scatterers/mie.jl:129-216.
  * q[5] = (q_ext, q_sca, q_abs, q_back, asymm).  S1, S2: [2*n_theta]
  * interleaved complex.  F11..F34: [n_theta].  Any output may be NULL. */
 int mcrt_bhmie(double x, double m_re, double m_im, int32_t n_theta, int32_t nextra,
@@ -196,7 +196,7 @@ int mcrt_bhmie(double x, double m_re, double m_im, int32_t n_theta, int32_t next
                double* F12, double* F33, double* F34);
 
 /* calculate_csrn(linear_interpolation(range(0, π, Nθ), F11), random_points)
- * — src/utils.jl:263-287 with src/scatterers/mie.jl:48-49.  csrn has
+ * — src/utils.jl:263-287 with src/scatterers/mie.jl:41-42.  csrn has
  * random_points+1 values. */
 int mcrt_calculate_csrn(const double* F11, int32_t n_theta, int32_t random_points,
                         double* csrn);
@@ -271,6 +271,41 @@ void mcrt_plan_destroy(mcrt_plan* plan);
  * the solve ran as a single batch.  Any output may be NULL. */
 int mcrt_plan_read_sigma(mcrt_plan* plan, double A_sigma[5], double* Irt_sigma,
                          double* Icb_sigma, int32_t* batches);
+
+/* Known-answer evaluation of the plan's own device building blocks on n fixed
+ * inputs: the functions the plan's transport kernel calls, in its arithmetic
+ * (FP32 for MIXED, FP64 otherwise), with its scatterer tables and geometry.
+ * in: n records of 16 doubles; out: n records of 4 doubles (unused words 0).
+ * Blocking.  No reference counterpart: it exposes, for checks against the
+ * reference's FP64 functions, what `simulate` only uses internally.
+ *   UPDATE_I    in (cosθ, sinθ, cosϕ, sinϕ, I[4], k[3]) → update_I(I, k)
+ *               (src/utils.jl:91-112) as the peel-off evaluates it
+ *   UPDATE_E    in (frame[4], E₁ re, E₁ im, E₂ re, E₂ im, k[3]) → update_E
+ *               (utils.jl:170-192) as the CB bins evaluate it
+ *   UPDATE_E_PATH  the same, as the FP64 CB path state evaluates it
+ *   DISTANCE    in (x[3], k[3], layer index) → (distance_to_boundary in the
+ *               form's arithmetic, the same in FP64) (plane.jl:97-105,
+ *               sphere.jl:69-73)
+ *   PHASE       in (cosθ) → (P11, P12, P33, P34) of the peel-off's lookup
+ *               (rayleigh.jl:12-17, mie.jl:53-71)
+ *   AMPLITUDE   in (cosθ) → (S11 re, im, S22 re, im) (rayleigh.jl:20-23,
+ *               mie.jl:76-92)
+ *   SAMPLE      in (u) → cosθ drawn by inversion (rayleigh.jl:9-11, mie.jl:47-51)
+ *   PHASE_TRANSPORT  in (cosθ) → the phase matrix as the transport's scatter
+ *               evaluates it (utils.jl:298)
+ *   KEPLER      in (M, e) → kepler_solver(M, e) (utils.jl:243-261) */
+enum {
+  MCRT_EVAL_UPDATE_I = 0,
+  MCRT_EVAL_UPDATE_E = 1,
+  MCRT_EVAL_UPDATE_E_PATH = 2,
+  MCRT_EVAL_DISTANCE = 3,
+  MCRT_EVAL_PHASE = 4,
+  MCRT_EVAL_AMPLITUDE = 5,
+  MCRT_EVAL_SAMPLE = 6,
+  MCRT_EVAL_PHASE_TRANSPORT = 7,
+  MCRT_EVAL_KEPLER = 8
+};
+int mcrt_plan_eval(mcrt_plan* plan, int32_t what, int64_t n, const double* in, double* out);
 
 /* ---- batched solves ------------------------------------------------------- */
 

@@ -610,6 +610,38 @@ extern "C" int mcrt_plan_read_sigma(mcrt_plan* P, double A_sigma[5], double* Irt
   return MCRT_OK;
 }
 
+extern "C" int mcrt_plan_eval(mcrt_plan* P, int32_t what, int64_t n, const double* in, double* out) {
+  if (!P) return fail(nullptr, MCRT_ERR_INVALID, "plan is NULL");
+  mcrt_ctx* ctx = P->ctx;
+  if (what < MCRT_EVAL_UPDATE_I || what > MCRT_EVAL_KEPLER)
+    return fail(ctx, MCRT_ERR_INVALID, "plan_eval: unknown function %d", what);
+  if (n < 0 || (n > 0 && (!in || !out))) return fail(ctx, MCRT_ERR_INVALID, "plan_eval: bad buffers");
+  if (n == 0) return MCRT_OK;
+  for (int64_t i = 0; what == MCRT_EVAL_DISTANCE && i < n; ++i) {
+    const double idx = in[16 * i + 6];
+    const int nl = P->geo_kind == MCRT_GEOMETRY_PLANE ? (P->precision == MCRT_PRECISION_FP64_PLAIN
+                                                             ? P->plain.geo.nl : mcrt::kMaxLayers)
+                                                      : 1;
+    if (!(idx >= 0 && idx < nl && idx == (double)(int)idx))
+      return fail(ctx, MCRT_ERR_INVALID, "plan_eval: layer index %g out of range", idx);
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (P->precision != MCRT_PRECISION_FP64_PLAIN) {
+    int rc = mcrt::coop_eval(P->coop, ctx->stream, what, n, in, out, &ctx->err);
+    if (rc) return fail(ctx, rc, "plan_eval: %s", ctx->err.c_str());
+    return MCRT_OK;
+  }
+  DevBuf din, dout;
+  HIP_TRY(ctx, din.alloc(sizeof(double) * 16 * (size_t)n));
+  HIP_TRY(ctx, dout.alloc(sizeof(double) * 4 * (size_t)n));
+  HIP_TRY(ctx, hipMemcpyAsync(din.p, in, sizeof(double) * 16 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, mcrt::launch_plain_eval(P->dplain, what, n, din.as<double>(), dout.as<double>(), ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(out, dout.p, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return MCRT_OK;
+}
+
 extern "C" void mcrt_plan_destroy(mcrt_plan* P) {
   if (!P) return;
   (void)hipSetDevice(P->ctx->device);

@@ -123,6 +123,10 @@ hipError_t coop_prepare(CoopPlan& cp, hipStream_t stream);
 hipError_t coop_launch(CoopPlan& cp, hipStream_t stream);
 // Enqueue the reduction of the slabs into A, Irt, Icb.
 hipError_t coop_reduce(CoopPlan& cp, hipStream_t stream);
+// mcrt_plan_eval for a cooperative plan: the building blocks of its kernels
+// on n host records (16 doubles in, 4 out); blocking on stream.
+int coop_eval(const CoopPlan& cp, hipStream_t stream, int what, int64_t n, const double* in,
+              double* out, std::string* err);
 // Enqueue the batch-means standard error of the N-normalised outputs
 // (sig[nout] in the slab order: A[5], Irt, Icb).
 hipError_t coop_sigma(CoopPlan& cp, hipStream_t stream, int64_t N, double* sig);

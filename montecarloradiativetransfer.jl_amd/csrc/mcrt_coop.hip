@@ -159,7 +159,7 @@ MCRT_DEV void kahan(double* s, double* c, double x) {
 }
 
 // Table coordinate sn = acos(μ)·(Nθ-1)/π of a phase / amplitude lookup
-// (mie.jl:60-62).  FP32: Abramowitz & Stegun 4.4.46 (Hastings),
+// (mie.jl:53-55).  FP32: Abramowitz & Stegun 4.4.46 (Hastings),
 // acos(a) = √(1-a)·p(a) on [0, 1] with |error| ≤ 2e-8, its coefficients
 // pre-multiplied by (Nθ-1)/π, and acos(-a) = π - acos(a).  FP64: the library acos.
 template <typename R> MCRT_DEV R theta_index(const CoopParams<R>& p, R mu);
@@ -195,7 +195,7 @@ MCRT_DEV int node_of(double sn, double imax, double& f) {
 
 // Phase matrix (P11, P12, P33, P34) at cosθ = mu: Rayleigh (rayleigh.jl:12-17)
 // or a lookup in the padded LDS table F, linear in θ between nodes i0 and
-// i0 + 1 (mie.jl:60-80).
+// i0 + 1 (mie.jl:53-71).
 template <typename R, bool MIE>
 MCRT_DEV void phase_tab(const CoopParams<R>& p, const R* F, R mu, R P[4]) {
   if constexpr (!MIE) {
@@ -213,7 +213,7 @@ MCRT_DEV void phase_tab(const CoopParams<R>& p, const R* F, R mu, R P[4]) {
   }
 }
 
-// Amplitude matrix diag(S11, S22) (rayleigh.jl:20-23; mie.jl:83-99 with its
+// Amplitude matrix diag(S11, S22) (rayleigh.jl:20-23; mie.jl:76-92 with its
 // weights (msn+1-sn, sn-msn) = (2-f, f-1) on nodes i0, i0 + 1: the zero pad
 // node reproduces the skipped term at θ = π).
 template <typename R, bool MIE>
@@ -1191,6 +1191,79 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(const BatchChunk* __res
   }
 }
 
+// Known-answer evaluation (mcrt_plan_eval): the building blocks above on n
+// fixed records, converted to R on entry.  UPDATE_I/UPDATE_E records carry the
+// bin constants of k in words 11..15 (Eh₁, Eh₂, Eh × k), as the bin tables do.
+template <typename R, bool MIE>
+__global__ void eval_kernel(const CoopParams<R>* __restrict__ pp, int what, int64_t n,
+                            const double* __restrict__ in, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CoopParams<R>& p = *pp;
+  const double* a = in + 16 * i;
+  double* o = out + 4 * i;
+  R r[4] = {R(0), R(0), R(0), R(0)};
+  double d[4] = {0.0, 0.0, 0.0, 0.0};
+  bool dbl = false;
+  switch (what) {
+    case MCRT_EVAL_UPDATE_I:
+    case MCRT_EVAL_UPDATE_E: {
+      R m[8];
+      frame_matrix(Fr<R>{(R)a[0], (R)a[1], (R)a[2], (R)a[3]}, m);
+      const Plane<R> q = scat_plane(m, (R)a[8], (R)a[9], (R)a[10], (R)a[11], (R)a[12], (R)a[13],
+                                    (R)a[14], (R)a[15]);
+      if (what == MCRT_EVAL_UPDATE_I) {
+        const R I[4] = {(R)a[4], (R)a[5], (R)a[6], (R)a[7]};
+        const R J[2] = {R(2) * I[1], R(-2) * I[2]};
+        peel_I<R, MIE>(p, p.sca.F, I, J, q, r);
+        r[1] = r[1] * q.inv;
+        r[2] = r[2] * q.inv;
+      } else {
+        Cx<R> E1 = {(R)a[4], (R)a[5]}, E2 = {(R)a[6], (R)a[7]};
+        peel_E<R, MIE>(p, p.sca.S, q, E1, E2);
+        r[0] = E1.re * q.inv; r[1] = E1.im * q.inv; r[2] = E2.re * q.inv; r[3] = E2.im * q.inv;
+      }
+      break;
+    }
+    case MCRT_EVAL_UPDATE_E_PATH: {
+      Cx<double> E1 = {a[4], a[5]}, E2 = {a[6], a[7]};
+      update_E(p.sca64, E1, E2, V3<double>{a[8], a[9], a[10]}, Fr<double>{a[0], a[1], a[2], a[3]});
+      d[0] = E1.re; d[1] = E1.im; d[2] = E2.re; d[3] = E2.im;
+      dbl = true;
+      break;
+    }
+    case MCRT_EVAL_DISTANCE: {
+      const int idx = (int)a[6];
+      const Geo<R> geo{&p};
+      d[0] = (double)geo.dist(V3<R>{(R)a[0], (R)a[1], (R)a[2]}, V3<R>{(R)a[3], (R)a[4], (R)a[5]}, idx);
+      d[1] = dist64(p, V3<double>{a[0], a[1], a[2]}, V3<double>{a[3], a[4], a[5]}, idx);
+      dbl = true;
+      break;
+    }
+    case MCRT_EVAL_PHASE:
+      phase_tab<R, MIE>(p, p.sca.F, (R)a[0], r);
+      break;
+    case MCRT_EVAL_AMPLITUDE: {
+      Cx<R> S11, S22;
+      amp_tab<R, MIE>(p, p.sca.S, (R)a[0], S11, S22);
+      r[0] = S11.re; r[1] = S11.im; r[2] = S22.re; r[3] = S22.im;
+      break;
+    }
+    case MCRT_EVAL_SAMPLE:
+      r[0] = phase_sample(p.sca, (R)a[0]);
+      break;
+    case MCRT_EVAL_PHASE_TRANSPORT:
+      phase_eval(p.sca, (R)a[0], r);
+      break;
+    case MCRT_EVAL_KEPLER:
+      r[0] = kepler((R)a[0], (R)a[1]);
+      break;
+    default:
+      break;
+  }
+  for (int c = 0; c < 4; ++c) o[c] = dbl ? d[c] : (double)r[c];
+}
+
 // The value of output i in one private slab: output words A and Irt map
 // one-to-one, a CB word is the sum over the slab's CB groups.
 MCRT_DEV double slab_value(const double* sl, int64_t sn, int64_t i, int64_t nrt, int64_t ncb,
@@ -1967,6 +2040,51 @@ hipError_t coop_sigma(CoopPlan& cp, hipStream_t s, int64_t N, double* sig) {
   hipLaunchKernelGGL(slab_sigma, dim3(bl), dim3(th), 0, s, (const double*)cp.slabs, cp.nslab,
                      cp.slab_words, cp.nout, N, cp.nrt, cp.ncb, cp.cb_groups, sig);
   return hipGetLastError();
+}
+
+int coop_eval(const CoopPlan& cp, hipStream_t s, int what, int64_t n, const double* in,
+              double* out, std::string* err) {
+  if (n <= 0) return MCRT_OK;
+  std::vector<double> h(in, in + 16 * (size_t)n);
+  if (what == MCRT_EVAL_UPDATE_I || what == MCRT_EVAL_UPDATE_E) {
+    // the bin constants of k, as init_params tabulates them
+    for (int64_t i = 0; i < n; ++i) {
+      double* a = &h[16 * (size_t)i];
+      double Eh[3], Ev[3];
+      h_get_basis(a + 8, Eh, Ev);
+      a[11] = Eh[0]; a[12] = Eh[1];
+      a[13] = Eh[1] * a[10]; a[14] = -Eh[0] * a[10]; a[15] = Eh[0] * a[9] - Eh[1] * a[8];
+    }
+  }
+  void *din = nullptr, *dout = nullptr;
+  hipError_t e = hipMalloc(&din, sizeof(double) * h.size());
+  if (e == hipSuccess) e = hipMalloc(&dout, sizeof(double) * 4 * (size_t)n);
+  if (e == hipSuccess) e = hipMemcpyAsync(din, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    const int th = 256;
+    const int bl = (int)((n + th - 1) / th);
+    const double* di = (const double*)din;
+    double* dd = (double*)dout;
+    if (cp.mixed) {
+      const auto* P = (const CoopParams<float>*)cp.params_dev;
+      if (cp.mie) hipLaunchKernelGGL((eval_kernel<float, true>), dim3(bl), dim3(th), 0, s, P, what, n, di, dd);
+      else hipLaunchKernelGGL((eval_kernel<float, false>), dim3(bl), dim3(th), 0, s, P, what, n, di, dd);
+    } else {
+      const auto* P = (const CoopParams<double>*)cp.params_dev;
+      if (cp.mie) hipLaunchKernelGGL((eval_kernel<double, true>), dim3(bl), dim3(th), 0, s, P, what, n, di, dd);
+      else hipLaunchKernelGGL((eval_kernel<double, false>), dim3(bl), dim3(th), 0, s, P, what, n, di, dd);
+    }
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  if (e != hipSuccess) {
+    *err = hipGetErrorString(e);
+    return MCRT_ERR_HIP;
+  }
+  return MCRT_OK;
 }
 
 hipError_t coop_reduce(CoopPlan& cp, hipStream_t s) {

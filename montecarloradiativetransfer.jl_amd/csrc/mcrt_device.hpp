@@ -7,7 +7,7 @@
 //   get_basis 52-62, rotate 82-84, update_I 91-112, E_from_I 119-150,
 //   I_from_E 157-163, update_E 170-192, kepler_solver 243-261,
 //   scatter 294-323; Rayleigh phase/amplitude (scatterers/rayleigh.jl:8-23);
-//   Mie phase/amplitude lookup (scatterers/mie.jl:53-99).
+//   Mie phase/amplitude lookup (scatterers/mie.jl:46-92).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -153,7 +153,7 @@ template <typename R> struct ScaView {
   const R* S;
 };
 
-// Inverse-CDF sample of cosθ from the uniform u (rayleigh.jl:9-11, mie.jl:54-58).
+// Inverse-CDF sample of cosθ from the uniform u (rayleigh.jl:9-11, mie.jl:47-51).
 template <typename R> MCRT_DEV R phase_sample(const ScaView<R>& s, R u) {
   if (s.kind == 0) {
     R rn = R(2) * (R(1) - R(2) * u);
@@ -167,7 +167,7 @@ template <typename R> MCRT_DEV R phase_sample(const ScaView<R>& s, R u) {
 }
 
 // Phase matrix at cosθ = mu: P = (P11, P12, P33, P34) (rayleigh.jl:12-17,
-// mie.jl:60-80).  The Mie branch clamps |mu| ≤ 1 before acos.
+// mie.jl:53-71).  The Mie branch clamps |mu| ≤ 1 before acos.
 template <typename R> MCRT_DEV void phase_eval(const ScaView<R>& s, R mu, R P[4]) {
   if (s.kind == 0) {
     R m2 = mu * mu;
@@ -191,7 +191,7 @@ template <typename R> MCRT_DEV void phase_eval(const ScaView<R>& s, R mu, R P[4]
 }
 
 // Amplitude matrix diag(S11, S22) at cosθ = mu (rayleigh.jl:20-23,
-// mie.jl:83-99 with its one-step-shifted weights (msn+1-sn, sn-msn)).
+// mie.jl:76-92 with its one-step-shifted weights (msn+1-sn, sn-msn)).
 template <typename R> MCRT_DEV void amplitude(const ScaView<R>& s, R mu, Cx<R>& S11, Cx<R>& S22) {
   if (s.kind == 0) {
     S11 = {R(0), R(1)};

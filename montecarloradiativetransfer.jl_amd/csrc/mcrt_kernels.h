@@ -75,5 +75,9 @@ GeoDev make_geo(const struct mcrt_geometry* g, int* layer_slot);
 
 // p_dev: the parameter block in device memory.
 hipError_t launch_plain(const PlainParams* p_dev, int blocks, int threads, hipStream_t stream);
+// mcrt_plan_eval for the plain form: din/dout are device buffers of n records
+// (16 doubles in, 4 out).
+hipError_t launch_plain_eval(const PlainParams* p_dev, int what, int64_t n, const double* din,
+                             double* dout, hipStream_t stream);
 
 }  // namespace mcrt

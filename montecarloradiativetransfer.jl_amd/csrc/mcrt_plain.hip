@@ -355,7 +355,66 @@ __global__ __launch_bounds__(256) void plain_kernel(const PlainParams* __restric
     if (cnt[q]) atomicAdd(p.counters + q, cnt[q]);
 }
 
+// Known-answer evaluation (mcrt_plan_eval) of the functions plain_kernel
+// calls, with layer 0's scatterer.
+__global__ void plain_eval_kernel(const PlainParams* __restrict__ pp, int what, int64_t n,
+                                  const double* __restrict__ in, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const PlainParams& p = *pp;
+  const ScaView<R>& sca = p.sca[p.geo.sca[0]];
+  const double* a = in + 16 * i;
+  double* o = out + 4 * i;
+  R r[4] = {0.0, 0.0, 0.0, 0.0};
+  const Fr<R> f = {a[0], a[1], a[2], a[3]};
+  switch (what) {
+    case MCRT_EVAL_UPDATE_I: {
+      const R I[4] = {a[4], a[5], a[6], a[7]};
+      update_I(sca, I, V3<R>{a[8], a[9], a[10]}, f, r);
+      break;
+    }
+    case MCRT_EVAL_UPDATE_E:
+    case MCRT_EVAL_UPDATE_E_PATH: {
+      Cx<R> E1 = {a[4], a[5]}, E2 = {a[6], a[7]};
+      update_E(sca, E1, E2, V3<R>{a[8], a[9], a[10]}, f);
+      r[0] = E1.re; r[1] = E1.im; r[2] = E2.re; r[3] = E2.im;
+      break;
+    }
+    case MCRT_EVAL_DISTANCE:
+      r[0] = distance(p.geo, V3<R>{a[0], a[1], a[2]}, V3<R>{a[3], a[4], a[5]}, (int)a[6]);
+      r[1] = r[0];
+      break;
+    case MCRT_EVAL_PHASE:
+    case MCRT_EVAL_PHASE_TRANSPORT:
+      phase_eval(sca, a[0], r);
+      break;
+    case MCRT_EVAL_AMPLITUDE: {
+      Cx<R> S11, S22;
+      amplitude(sca, a[0], S11, S22);
+      r[0] = S11.re; r[1] = S11.im; r[2] = S22.re; r[3] = S22.im;
+      break;
+    }
+    case MCRT_EVAL_SAMPLE:
+      r[0] = phase_sample(sca, a[0]);
+      break;
+    case MCRT_EVAL_KEPLER:
+      r[0] = kepler(a[0], a[1]);
+      break;
+    default:
+      break;
+  }
+  for (int c = 0; c < 4; ++c) o[c] = r[c];
+}
+
 }  // namespace
+
+hipError_t launch_plain_eval(const PlainParams* p_dev, int what, int64_t n, const double* din,
+                             double* dout, hipStream_t stream) {
+  const int th = 256;
+  hipLaunchKernelGGL(plain_eval_kernel, dim3((unsigned)((n + th - 1) / th)), dim3(th), 0, stream,
+                     p_dev, what, n, din, dout);
+  return hipGetLastError();
+}
 
 hipError_t launch_plain(const PlainParams* p_dev, int blocks, int threads, hipStream_t stream) {
   hipLaunchKernelGGL(plain_kernel, dim3(blocks), dim3(threads), 0, stream, p_dev);

@@ -1,8 +1,8 @@
 // mcrt_tables.cpp — host FP64 scatterer-table and quadrature builders of
 // libmcrt_hip.so (no device use).
 //
-//   mcrt_bhmie            src/scatterers/mie.jl:136-227  (Bohren–Huffman series)
-//   mcrt_calculate_csrn   src/utils.jl:263-287 with src/scatterers/mie.jl:48-49
+//   mcrt_bhmie            src/scatterers/mie.jl:129-216  (Bohren–Huffman series)
+//   mcrt_calculate_csrn   src/utils.jl:263-287 with src/scatterers/mie.jl:41-42
 //   mcrt_gauss_legendre   FastGaussQuadrature.gausslegendre (plane.jl:132, sphere.jl:86)
 //   mcrt_initial_weights  src/geometries/plane.jl:131-142, sphere.jl:85-90
 //

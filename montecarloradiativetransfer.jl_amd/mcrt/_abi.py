@@ -159,6 +159,10 @@ class mcrt_server_stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+# mcrt_plan_eval functions (include/mcrt.h MCRT_EVAL_*)
+EVAL = {"update_I": 0, "update_E": 1, "update_E_path": 2, "distance": 3, "phase": 4,
+        "amplitude": 5, "sample": 6, "phase_transport": 7, "kepler": 8}
+
 TICKET_STATES = {0: "queued", 1: "running", 2: "done", 3: "cancelled", 4: "failed"}
 
 # Every function declared in include/mcrt.h: name -> (restype, argtypes).
@@ -194,6 +198,7 @@ SIGNATURES = {
                                           C.c_uint64, C.c_int32, C.c_uint64, C.c_uint32,
                                           C.POINTER(mcrt_launch_shape), C.POINTER(_vp)]),
     "mcrt_plan_shape": (C.c_int, [_vp, C.POINTER(mcrt_launch_shape), C.POINTER(C.c_int32)]),
+    "mcrt_plan_eval": (C.c_int, [_vp, C.c_int32, C.c_int64, _dp, _dp]),
     "mcrt_batch_create": (C.c_int, [_vp, C.c_int32, C.POINTER(mcrt_request), C.POINTER(_vp)]),
     "mcrt_batch_launch": (C.c_int, [_vp]),
     "mcrt_batch_wait": (C.c_int, [_vp]),

@@ -147,6 +147,17 @@ class Plan:
     def launch(self) -> None:
         _abi.check(self.ctx.lib.mcrt_plan_launch(self.handle), self.ctx.handle)
 
+    def eval(self, what: str, records) -> np.ndarray:
+        """mcrt_plan_eval: the plan's device building block `what` (a key of
+        _abi.EVAL) on records of up to 16 inputs each; returns (n, 4)."""
+        rec = np.atleast_2d(np.asarray(records, dtype=np.float64))
+        inp = np.zeros((rec.shape[0], 16))
+        inp[:, :rec.shape[1]] = rec
+        out = np.zeros((rec.shape[0], 4))
+        _abi.check(self.ctx.lib.mcrt_plan_eval(self.handle, _abi.EVAL[what], rec.shape[0],
+                                               _abi.dptr(inp), _abi.dptr(out)), self.ctx.handle)
+        return out
+
     def wait(self) -> None:
         _abi.check(self.ctx.lib.mcrt_plan_wait(self.handle), self.ctx.handle)
 
@@ -195,9 +206,10 @@ def mueller_sphere(cfg: Config, runs):
     """Mueller assembly of run_rt(::SphereGeometry), :87-184, from simulate outputs.
 
     runs: list of (I0, simulate-tuple) for the columns of initial_I(s).
-    Returns (MRT[4,4,Nθ], MCB[4,4,Nθb], energy dict)."""
-    if cfg.Nphi % 8 or cfg.Nphib % 8:
-        raise ValueError("the sphere Mueller assembly needs Nϕ and Nϕb divisible by 8")
+    Returns (MRT[4,4,Nθ], MCB[4,4,Nθb], energy dict).  The azimuth columns are
+    NFI[i] = i·Nϕ÷8 (0-based) for any Nϕ, as the reference's i·Nϕ÷8+1 (:94-95):
+    the assembly is exact when Nϕ is a multiple of 8 and uses the nearest lower
+    columns otherwise."""
     MRT = np.zeros((4, 4, cfg.Ntheta))
     MCB = np.zeros((4, 4, cfg.Nthetab))
     NFI = [i * cfg.Nphi // 8 for i in range(8)]

@@ -100,6 +100,10 @@ class Config:
         if "Nphib" in kw:
             for k in ("phib", "cosphib", "sinphib"):
                 d.pop(k)
+        if "phib" in kw:
+            for k in ("Nphib", "cosphib", "sinphib"):
+                d.pop(k, None)
+            kw.setdefault("Nphib", len(kw["phib"]))
         d.update(kw)
         return Config(geometry, **d)
 

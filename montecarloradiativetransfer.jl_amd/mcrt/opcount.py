@@ -15,9 +15,9 @@ index arithmetic count 0; dense StaticArrays products count their zero terms
   rotate + angles      utils.jl:108-109  (3 + 2 + 6) × 2                        =  22
   dense 4×4·4          utils.jl:108      16 mul + 12 add                        =  28
   phase matrix         rayleigh.jl:12-17 μ², 1±μ² (2), ×0.75 (2), 1.5μ         =   6
-                       mie.jl:60-73      acos, /π, ×(N-1), w 3, 8 mul, 4 add    =  18
+                       mie.jl:53-71      acos, /π, ×(N-1), w 3, 8 mul, 4 add    =  18
   amplitude matrix     rayleigh.jl:21    μ·1.0im                                 =   2
-                       mie.jl:84-93      acos, /π, ×(N-1), w 3, 8 mul, 4 add    =  18
+                       mie.jl:77-86      acos, /π, ×(N-1), w 3, 8 mul, 4 add    =  18
   update_E             utils.jl:170-192  frame 77 + ϕ-rotation 12 + S·E (4 cmul
                                          24 + 2 cadd 4) + ψ-rotation 12 + S      = 129 + S
   I_from_E             utils.jl:157-163  3 + 3 + 2 cmul 12 + 2 + 2 + cmul 6 + 2 =  30

@@ -1,7 +1,7 @@
 """Scatterers: RayleighScatterer and MieScatterer.
 
 Mirrors src/scatterers/index.jl (mean_free_path, single_scattering_albedo),
-src/scatterers/rayleigh.jl:3-6 and src/scatterers/mie.jl:3-51.  The Mie
+src/scatterers/rayleigh.jl:3-6 and src/scatterers/mie.jl:3-44.  The Mie
 tables come from the library's FP64 builders (``mcrt_bhmie``,
 ``mcrt_calculate_csrn``), the same arithmetic the device tables are staged
 from.  SplineScatterer (src/scatterers/spline.jl) is not provided: its
@@ -45,7 +45,7 @@ class RayleighScatterer(AbstractScatterer):
 
 @dataclass(frozen=True)
 class BHMieResult:
-    """Output of bhmie (src/scatterers/mie.jl:221-222)."""
+    """Output of bhmie (src/scatterers/mie.jl:214-215)."""
 
     q_ext: float
     q_sca: float
@@ -69,7 +69,7 @@ class BHMieResult:
 
 
 def bhmie(x: float, m: complex, Ntheta: int = 181, nextra: int = 15, custom_nstop: int = 0) -> BHMieResult:
-    """bhmie(x, m; Nθ, nextra, custom_nstop) — src/scatterers/mie.jl:136-227 (FP64)."""
+    """bhmie(x, m; Nθ, nextra, custom_nstop) — src/scatterers/mie.jl:129-216 (FP64)."""
     lib = _abi.load_library()
     m = complex(m)
     N = int(Ntheta)
@@ -95,7 +95,7 @@ def calculate_csrn(F11: np.ndarray, random_points: int) -> np.ndarray:
 
 @dataclass(frozen=True, eq=False)
 class MieScatterer(AbstractScatterer):
-    """MieScatterer — src/scatterers/mie.jl:3-15 (struct) and :43-51 (constructor).
+    """MieScatterer — src/scatterers/mie.jl:3-15 (struct) and :36-44 (constructor).
 
     Build it with ``MieScatterer.create(r, m, λ, ν; Ntheta, random_points)``
     (the reference's ``MieScatterer(r, m, λ, ν; Nθ, random_points)``), or pass

@@ -178,7 +178,7 @@ void or_rotate(const double I[4], double c2, double s2, double o[4]) {
 /* Scatterers — src/scatterers/rayleigh.jl, mie.jl                           */
 /* ------------------------------------------------------------------------ */
 /* Inverse-CDF sample of cosθ given the uniform u:
- * Rayleigh rayleigh.jl:9-11 (Cardano), Mie mie.jl:54-58 (csrn lerp). */
+ * Rayleigh rayleigh.jl:9-11 (Cardano), Mie mie.jl:47-51 (csrn lerp). */
 double or_phase_sample(const mcrt_scatterer* s, double u) {
   if (s->kind == MCRT_SCATTERER_RAYLEIGH) {
     double rn = 2.0 * (1.0 - 2.0 * u);
@@ -192,7 +192,7 @@ double or_phase_sample(const mcrt_scatterer* s, double u) {
   }
 }
 
-/* phase_matrix at a given cosθ: Rayleigh rayleigh.jl:12-17, Mie mie.jl:60-80.
+/* phase_matrix at a given cosθ: Rayleigh rayleigh.jl:12-17, Mie mie.jl:53-71.
  * P[0..3] = (P11, P12, P33, P34); P22 = P11, P44 = P33, P43 = -P34. */
 void or_phase_eval(const mcrt_scatterer* s, double mu, double P[4]) {
   if (s->kind == MCRT_SCATTERER_RAYLEIGH) {
@@ -222,7 +222,7 @@ void or_phase_eval(const mcrt_scatterer* s, double mu, double P[4]) {
   }
 }
 
-/* amplitude_matrix: Rayleigh rayleigh.jl:20-23; Mie mie.jl:83-99 including
+/* amplitude_matrix: Rayleigh rayleigh.jl:20-23; Mie mie.jl:76-92 including
  * its one-step-shifted interpolation weights (msn+1-sn, sn-msn). */
 void or_amplitude(const mcrt_scatterer* s, double mu, or_cplx S[2]) {
   if (s->kind == MCRT_SCATTERER_RAYLEIGH) {
@@ -423,7 +423,7 @@ static double range_theta(int i, int N) {
   return (double)((long double)OR_PI * (long double)i / (long double)(N - 1));
 }
 
-/* bhmie, src/scatterers/mie.jl:136-227 */
+/* bhmie, src/scatterers/mie.jl:129-216 */
 int or_bhmie(double x, double m_re, double m_im, int Nt, int nextra, int custom_nstop,
              double q[5], double* S1o, double* S2o, double* F11, double* F12, double* F33,
              double* F34) {
@@ -1095,6 +1095,27 @@ int or_trace_ray(const mcrt_config* cfg, const mcrt_geometry* g, const double I0
   or_rng rng = {seed, ray, instance, 0};
   trace_one(&T, &rng, A, Irt, Icb, counters);
   free(T.kpath); free(T.xpath); free(T.idxpath);
+  return MCRT_OK;
+}
+
+int or_collect_cb_event(const mcrt_config* cfg, const mcrt_geometry* g, const double I0[4],
+                        const double I[4], const double k[3], const double x[3], int idx,
+                        int64_t n, const double* kpath, const double* xpath, const int* idxpath,
+                        double* Icb) {
+  geo_t G;
+  int rc = geo_init(g, &G);
+  if (rc) return rc;
+  if ((rc = check_cfg(cfg))) return rc;
+  if (!cfg->compute_cb || n < 1 || !kpath || !xpath || !idxpath || !Icb) return MCRT_ERR_INVALID;
+  tctx_t T;
+  T.cfg = cfg; T.G = &G;
+  memcpy(T.I0, I0, sizeof T.I0);
+  incident_frame(g, T.k0, T.f0);
+  T.kpath = (double*)kpath;
+  T.xpath = (double*)xpath;
+  T.idxpath = (int*)idxpath;
+  uint64_t cnt[6] = {0, 0, 0, 0, 0, 0};
+  collect_cb(&T, Icb, I, k, x, idx, n, &cnt[2]);
   return MCRT_OK;
 }
 

@@ -71,6 +71,16 @@ int or_trace_ray(const mcrt_config* cfg, const mcrt_geometry* g, const double I0
                  uint64_t seed, uint32_t instance, uint64_t ray, double A[5], double* Irt,
                  double* Icb, uint64_t counters[6]);
 
+/* collect_cb_cpu! (src/cpu.jl:38-160) for one event, for the checks of its
+ * n = 1 and n > 1 bodies: I is the event's Stokes vector after absorption, k
+ * the direction arriving at x, n the event number; kpath/xpath/idxpath hold
+ * the path's first n directions, positions and 0-based layers (kpath[0] must
+ * be the incident direction, as in cpu_kernel).  Adds into Icb[4*Nθb*Nϕb]. */
+int or_collect_cb_event(const mcrt_config* cfg, const mcrt_geometry* g, const double I0[4],
+                        const double I[4], const double k[3], const double x[3], int idx,
+                        int64_t n, const double* kpath, const double* xpath, const int* idxpath,
+                        double* Icb);
+
 /* simulate's CPU branch (MonteCarloRadiativeTransfer.jl:57-76) over rays
  * [ray_begin, ray_end), split into nbatch contiguous equal batches.  Each
  * batch's raw sums go to batches[b*(5+nrt+ncb) ...] (A, Irt, Icb), so the

@@ -70,6 +70,8 @@ def lib():
                                    _dp, _dp, _dp, C.POINTER(C.c_uint64)]
         L.or_simulate.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_uint64, C.c_uint32, C.c_int64,
                                   C.c_int64, C.c_int, C.c_int, _dp, C.POINTER(C.c_uint64)]
+        L.or_collect_cb_event.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, _dp, _dp, C.c_int,
+                                          C.c_int64, _dp, _dp, C.POINTER(C.c_int), _dp]
         _lib = L
     return _lib
 
@@ -292,6 +294,22 @@ class Solve:
                                 Irt.ctypes.data_as(_dp), Icb.ctypes.data_as(_dp), cnt)
         assert rc == 0, rc
         return A, Irt, Icb, list(cnt)
+
+    def collect_cb_event(self, I0, I, k, x, idx, kpath, xpath, idxpath=None):
+        """collect_cb_cpu! (cpu.jl:38-160) for one event: the event number is
+        len(kpath); returns Icb[4, Nθb, Nϕb] of this event alone."""
+        cfg = self.cfg
+        kp = np.ascontiguousarray(np.asarray(kpath, dtype=np.float64).reshape(-1))
+        xp = np.ascontiguousarray(np.asarray(xpath, dtype=np.float64).reshape(-1))
+        n = len(kp) // 3
+        ip = (C.c_int * n)(*(idxpath if idxpath is not None else [0] * n))
+        Icb = np.zeros(4 * cfg.Nthetab * cfg.Nphib)
+        a = [_d(v) for v in (I0, I, k, x)]
+        rc = lib().or_collect_cb_event(self.cptr, self.gptr, a[0][1], a[1][1], a[2][1], a[3][1], idx,
+                                       n, kp.ctypes.data_as(_dp), xp.ctypes.data_as(_dp), ip,
+                                       Icb.ctypes.data_as(_dp))
+        assert rc == 0, rc
+        return Icb.reshape(cfg.Nphib, cfg.Nthetab, 4).transpose(2, 1, 0)
 
     def simulate(self, I0, seed, n_rays=None, ray_begin=0, nthreads=None, nbatch=32, instance=0):
         """Raw batch sums: returns dict(A, Irt, Icb) totals normalised by N, per-batch

@@ -1,5 +1,5 @@
 """Write tests/golden/bhmie_golden.json: 50-digit mpmath evaluations of the
-Bohren–Huffman series exactly as src/scatterers/mie.jl:136-227 truncates it
+Bohren–Huffman series exactly as src/scatterers/mie.jl:129-216 truncates it
 (nstop = ⌊max(x + 4∛x + 2, |m|x)⌋, nmax = nstop + 15, D_n downward from
 D_nmax = 0, angles θ_i = i·π/(Nθ-1) with the mirrored half by parity).
 

@@ -1,0 +1,192 @@
+"""Device-level known-answer tests of the deterministic building blocks.
+
+Each form's own device functions (mcrt_plan_eval through the C ABI) are
+evaluated on fixed inputs and compared value for value with the oracle's FP64
+restatement of the reference functions:
+
+  update_I (utils.jl:91-112) as the peel-off evaluates it, update_E
+  (utils.jl:170-192) as the CB bins and as the FP64 CB path state evaluate it,
+  distance_to_boundary (plane.jl:97-105, sphere.jl:69-73), the phase and
+  amplitude lookups (rayleigh.jl:8-23, mie.jl:46-92), inverse-CDF sampling and
+  kepler_solver (utils.jl:243-261).
+
+Bounds: 1e-12 relative to the output's scale in the FP64 forms (fp64_plain
+repeats the reference's operations in order; fp64 evaluates the peel-off in
+the scattering-plane form, so it agrees to rounding).  The mixed form computes
+in FP32: its bound is FP32_REL (3e-5) of the output's scale, where the scale is
+max(1, max |reference|) — the FP32 arccos of the table lookup (|error| ≤ 2e-8
+rad, Abramowitz & Stegun 4.4.46) and the FP32 rounding of μ (≈ 6e-8/sinθ rad,
+≲ 2.5e-4 of a 0.1° table step for |μ| ≤ 0.99) dominate; distances are bounded
+at FP32_REL of the medium's size.  Inputs keep |μ| ≤ 0.99 for Mie lookups,
+where the lookup is well conditioned; the polar branch (μ = ±1 exactly) is
+exercised with Rayleigh.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import mcrt
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ["fp64_plain", "fp64", "mixed"]
+FP64_REL = 1e-12
+FP32_REL = 3e-5
+rng = np.random.default_rng(77)
+
+
+def _tol(form):
+    return FP32_REL if form == "mixed" else FP64_REL
+
+
+def _unit(n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1)[:, None]
+
+
+def _frames(n):
+    t = rng.uniform(0.02, math.pi - 0.02, n)
+    p = rng.uniform(0, 2 * math.pi, n)
+    return np.stack([np.cos(t), np.sin(t), np.cos(p), np.sin(p)], 1)
+
+
+def _kdir(f):
+    return np.array([f[1] * f[2], f[1] * f[3], f[0]])
+
+
+def _case(kind, geom):
+    sca = mcrt.RayleighScatterer(0.9, 1.3) if kind == "rayleigh" else mcrt.workloads.c2_mie(1801)
+    if geom == "plane":
+        g = mcrt.PlaneGeometry(0.8 * math.pi, 0.4, [mcrt.Layer(d=2.0 * sca.kl, scatterer=sca, tau=2.0),
+                                                      mcrt.Layer(d=1.0 * sca.kl, scatterer=sca, tau=1.0)])
+    else:
+        g = mcrt.SphereGeometry(R=3.0 * sca.kl, scatterer=sca)
+    cfg = mcrt.Config(g, number_of_rays=64, Ntheta=8, Nphi=8, compute_cb=True)
+    return sca, g, cfg
+
+
+def _directions(fr, mie, n):
+    """Bin directions k for the frames fr: random, with |μ| ≤ 0.99 for Mie; for
+    Rayleigh a few exactly along ±k of the frame (the polar branch)."""
+    ks = []
+    for i in range(n):
+        kin = _kdir(fr[i])
+        if not mie and i % 50 == 0:
+            ks.append(kin if i % 100 == 0 else -kin)
+            continue
+        while True:
+            k = _unit(1)[0]
+            if abs(k @ kin) <= 0.99:
+                break
+        ks.append(k)
+    return np.array(ks)
+
+
+def _check(form, got, want, label, scale=None):
+    got = np.asarray(got, dtype=float)
+    want = np.asarray(want, dtype=float)
+    sc = np.maximum(1.0, np.abs(want).max(axis=-1)) if scale is None else scale
+    err = np.abs(got - want).max(axis=-1) / sc
+    worst = float(err.max())
+    print(f"kat {form} {label}: max rel err {worst:.3e} (bound {_tol(form):.0e})")
+    assert worst <= _tol(form), (form, label, worst, int(err.argmax()))
+    return worst
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("kind", ["rayleigh", "mie"])
+def test_kat_update_I_and_E(form, kind):
+    sca, g, cfg = _case(kind, "sphere")
+    osca = oracle.Sca(sca)
+    plan = mcrt.Plan(cfg, g, [1.0, 0.0, 0.0, 0.0], precision=form)
+    try:
+        n = 600
+        fr = _frames(n)
+        ks = _directions(fr, kind == "mie", n)
+        I = np.concatenate([np.ones((n, 1)), _unit(n) * rng.uniform(0, 1, (n, 1))], 1) * \
+            rng.uniform(0.1, 2.0, (n, 1))
+        E = rng.normal(size=(n, 4))
+        got = plan.eval("update_I", np.concatenate([fr, I, ks], 1))
+        want = np.array([oracle.update_I(osca, I[i], ks[i], fr[i])[0] for i in range(n)])
+        _check(form, got, want, f"{kind} update_I")
+        Ec = E[:, 0::2] + 1j * E[:, 1::2]
+        want_e = []
+        for i in range(n):
+            e, _, _ = oracle.update_E(osca, Ec[i], ks[i], fr[i])
+            want_e.append([e[0].real, e[0].imag, e[1].real, e[1].imag])
+        want_e = np.array(want_e)
+        got = plan.eval("update_E", np.concatenate([fr, E, ks], 1))
+        _check(form, got, want_e, f"{kind} update_E (bins)")
+        got = plan.eval("update_E_path", np.concatenate([fr, E, ks], 1))
+        # the CB path state is FP64 in every form
+        _check("fp64", got, want_e, f"{kind} update_E (path state, {form})")
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("geom", ["plane", "sphere"])
+def test_kat_distance(form, geom):
+    sca, g, cfg = _case("rayleigh", geom)
+    S = oracle.Solve(cfg, g)
+    plan = mcrt.Plan(cfg, g, [1.0, 0.0, 0.0, 0.0], precision=form)
+    try:
+        n = 800
+        ks = _unit(n)
+        if geom == "plane":
+            idx = rng.integers(0, 2, n)
+            tau = np.where(idx == 0, 2.0, 1.0)
+            x = np.stack([rng.uniform(-3, 3, n), rng.uniform(-3, 3, n), -rng.uniform(0, 1, n) * tau], 1)
+            size = 2.0
+        else:
+            idx = np.zeros(n, dtype=int)
+            x = _unit(n) * rng.uniform(0, 2.99, (n, 1))
+            size = 3.0
+        got = plan.eval("distance", np.concatenate([x, ks, idx[:, None]], 1))
+        want = np.array([S.distance(x[i], ks[i], int(idx[i])) for i in range(n)])
+        fin = np.isfinite(want)
+        assert np.array_equal(np.isfinite(got[:, 0]), fin)
+        _check(form, got[fin, :1], want[fin, None], f"{geom} distance", scale=size)
+        # the CB path state's FP64 distance
+        _check("fp64", got[fin, 1:2], want[fin, None], f"{geom} distance (FP64 path state, {form})",
+               scale=size)
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("kind", ["rayleigh", "mie"])
+def test_kat_lookups_sampling_kepler(form, kind):
+    sca, g, cfg = _case(kind, "sphere")
+    osca = oracle.Sca(sca)
+    plan = mcrt.Plan(cfg, g, [1.0, 0.0, 0.0, 0.0], precision=form)
+    try:
+        n = 2000
+        mu = rng.uniform(-0.99, 0.99, n)
+        if kind == "mie":  # table nodes and midpoints too
+            th = (np.arange(0, 1800, 9) + rng.choice([0.0, 0.5], 200)) * math.pi / 1800
+            mu = np.concatenate([mu, np.cos(th[np.abs(np.cos(th)) <= 0.99])])
+        else:
+            mu = np.concatenate([mu, [-1.0, 1.0, 0.0]])
+        P = np.array([oracle.phase_eval(osca, m) for m in mu])
+        _check(form, plan.eval("phase", mu[:, None]), P, f"{kind} phase (peel-off)")
+        _check(form, plan.eval("phase_transport", mu[:, None]), P, f"{kind} phase (transport)")
+        S = np.array([[v.real, v.imag, w.real, w.imag] for v, w in (oracle.amplitude(osca, m) for m in mu)])
+        _check(form, plan.eval("amplitude", mu[:, None]), S, f"{kind} amplitude")
+        u = np.concatenate([rng.uniform(1e-6, 1 - 1e-6, 1000), [0.5, 0.25, 0.75]])
+        want = np.array([oracle.phase_sample(osca, x) for x in u])
+        _check(form, plan.eval("sample", u[:, None])[:, :1], want[:, None], f"{kind} sample")
+        M = rng.uniform(0, 6 * math.pi, 300)
+        e = rng.uniform(0, 0.99, 300)
+        want = np.array([oracle.kepler(a, b) for a, b in zip(M, e)])
+        got = plan.eval("kepler", np.stack([M, e], 1))[:, :1]
+        if form == "mixed":  # FP32 Newton-type stop at 2e-6 (mcrt_device.hpp), FP32 M
+            assert np.max(np.abs(got[:, 0] - want) / np.maximum(1, np.abs(want))) < 1e-4
+        else:
+            _check(form, got, want[:, None], "kepler", scale=np.maximum(1, np.abs(want)))
+    finally:
+        plan.close()
