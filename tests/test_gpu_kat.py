@@ -13,13 +13,15 @@ restatement of the reference functions:
 Bounds: 1e-12 relative to the output's scale in the FP64 forms (fp64_plain
 repeats the reference's operations in order; fp64 evaluates the peel-off in
 the scattering-plane form, so it agrees to rounding).  The mixed form computes
-in FP32: its bound is FP32_REL (3e-5) of the output's scale, where the scale is
+in FP32: its bound is FP32_REL (1e-5) of the output's scale, where the scale is
 max(1, max |reference|) — the FP32 arccos of the table lookup (|error| ≤ 2e-8
 rad, Abramowitz & Stegun 4.4.46) and the FP32 rounding of μ (≈ 6e-8/sinθ rad,
 ≲ 2.5e-4 of a 0.1° table step for |μ| ≤ 0.99) dominate; distances are bounded
 at FP32_REL of the medium's size.  Inputs keep |μ| ≤ 0.99 for Mie lookups,
 where the lookup is well conditioned; the polar branch (μ = ±1 exactly) is
-exercised with Rayleigh.
+exercised with Rayleigh.  At the table nodes themselves the reference's
+amplitude lookup is discontinuous (its shifted weights), and either one-sided
+value is accepted there.
 """
 from __future__ import annotations
 
@@ -35,7 +37,7 @@ pytestmark = pytest.mark.gpu
 
 FORMS = ["fp64_plain", "fp64", "mixed"]
 FP64_REL = 1e-12
-FP32_REL = 3e-5
+FP32_REL = 1e-5
 rng = np.random.default_rng(77)
 
 
@@ -86,14 +88,17 @@ def _directions(fr, mie, n):
     return np.array(ks)
 
 
-def _check(form, got, want, label, scale=None):
+def _check(form, got, want, label, scale=None, inputs=None):
     got = np.asarray(got, dtype=float)
     want = np.asarray(want, dtype=float)
     sc = np.maximum(1.0, np.abs(want).max(axis=-1)) if scale is None else scale
     err = np.abs(got - want).max(axis=-1) / sc
     worst = float(err.max())
-    print(f"kat {form} {label}: max rel err {worst:.3e} (bound {_tol(form):.0e})")
-    assert worst <= _tol(form), (form, label, worst, int(err.argmax()))
+    w = int(err.argmax())
+    where = "" if inputs is None else f" at input {np.asarray(inputs)[w]}"
+    print(f"kat {form} {label}: max rel err {worst:.3e} (bound {_tol(form):.0e}){where} "
+          f"got {got[w]} want {want[w]}")
+    assert worst <= _tol(form), (form, label, worst, w)
     return worst
 
 
@@ -173,10 +178,26 @@ def test_kat_lookups_sampling_kepler(form, kind):
         else:
             mu = np.concatenate([mu, [-1.0, 1.0, 0.0]])
         P = np.array([oracle.phase_eval(osca, m) for m in mu])
-        _check(form, plan.eval("phase", mu[:, None]), P, f"{kind} phase (peel-off)")
-        _check(form, plan.eval("phase_transport", mu[:, None]), P, f"{kind} phase (transport)")
+        _check(form, plan.eval("phase", mu[:, None]), P, f"{kind} phase (peel-off)", inputs=mu)
+        _check(form, plan.eval("phase_transport", mu[:, None]), P, f"{kind} phase (transport)",
+               inputs=mu)
         S = np.array([[v.real, v.imag, w.real, w.imag] for v, w in (oracle.amplitude(osca, m) for m in mu)])
-        _check(form, plan.eval("amplitude", mu[:, None]), S, f"{kind} amplitude")
+        got = plan.eval("amplitude", mu[:, None])
+        if kind == "mie":
+            # The reference's amplitude weights (msn+1-sn, sn-msn) make the lookup
+            # jump at every node j: S[j-1] just below, 2S[j] - S[j+1] at and above
+            # (mie.jl:76-86).  At a node, rounding of arccos decides the side, so
+            # the device may take either one-sided value.
+            sn = np.arccos(mu) / math.pi * (sca.Ntheta - 1)
+            node = np.abs(sn - np.round(sn)) < 1e-9
+            j = np.round(sn).astype(int)
+            tab = np.stack([sca.S11.real, sca.S11.imag, sca.S22.real, sca.S22.imag], 1)
+            for i in np.nonzero(node)[0]:
+                lo = tab[j[i] - 1]
+                hi = 2 * tab[j[i]] - tab[j[i] + 1]
+                S[i] = lo if np.abs(got[i] - lo).max() < np.abs(got[i] - hi).max() else hi
+            assert node.sum() > 50  # the node cases were exercised
+        _check(form, got, S, f"{kind} amplitude", inputs=mu)
         u = np.concatenate([rng.uniform(1e-6, 1 - 1e-6, 1000), [0.5, 0.25, 0.75]])
         want = np.array([oracle.phase_sample(osca, x) for x in u])
         _check(form, plan.eval("sample", u[:, None])[:, :1], want[:, None], f"{kind} sample")
