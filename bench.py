@@ -52,6 +52,26 @@ def run_form(cfg, g, seed, I0, form, steps, warmup):
     return dict(wall_s=t1 - t0, kernel_ms=kms, stats=st, out=out)
 
 
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: the CPUs this process may run on, at
+    most OMP_NUM_THREADS when set (the box's CPU share) and at most 16."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, min(n, 16))
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def measured_traffic(workload: str, rays: int, form: str):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summaries (profiles/*.md lines `traffic_bytes_per_launch[workload, N rays,
@@ -97,12 +117,14 @@ def main() -> int:
     ap.add_argument("--form", default="auto", choices=["auto", "mixed", "fp64", "fp64_plain"])
     ap.add_argument("--compare", default="fp64,fp64_plain",
                     help="other forms timed beside the headline form")
-    ap.add_argument("--plain-sample", type=int, default=8192,
-                    help="rays of the plain FP64 comparison run")
+    ap.add_argument("--plain-sample", type=int, default=65536,
+                    help="rays of the plain FP64 comparison run (8 per thread at the "
+                         "reference's maximum_threads = 8192, so that it measures throughput "
+                         "rather than the slowest single ray)")
     ap.add_argument("--fp64-sample", type=int, default=2_000_000,
                     help="rays of the FP64 comparison run")
-    ap.add_argument("--cpu-sample", type=int, default=8192,
-                    help="rays of the oracle CPU baseline run")
+    ap.add_argument("--cpu-sample", type=int, default=4096,
+                    help="rays of each of the 3 oracle CPU baseline runs (the median is reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="seconds of Poisson serving load (200 req/s) for the p50/p99 latency; 0: skip")
@@ -173,16 +195,20 @@ def main() -> int:
     if not args.no_cpu:
         import oracle
 
-        nth = min(os.cpu_count() or 1, 16)
+        nth = cpu_threads()
         ncpu = min(args.cpu_sample, N)
         S = oracle.Solve(cfg.replace(number_of_rays=ncpu), g)
-        t0 = time.perf_counter()
-        S.simulate(I0, seed, nthreads=nth, nbatch=max(nth * 4, 16))
-        dt = time.perf_counter() - t0
+        times = []
+        for k in range(3):  # three runs over the same rays, the median reported
+            t0 = time.perf_counter()
+            S.simulate(I0, seed, nthreads=nth, nbatch=max(nth * 4, 16))
+            times.append(time.perf_counter() - t0)
+        dt = sorted(times)[1]
         cpu = {"value": ncpu / dt, "unit": "rays/s", "cores": nth, "kind": "port",
-               "sample": f"rays 0..{ncpu - 1} of the same workload and seed (oracle, FP64, "
-                         f"{nth} threads, {dt:.1f} s)"}
-        log(f"bench: cpu oracle {ncpu / dt:.4g} rays/s on {nth} threads")
+               "sample": f"rays 0..{ncpu - 1} of the same workload and seed, median of 3 runs "
+                         f"({', '.join(f'{t:.2f}' for t in times)} s; oracle, FP64, {nth} threads "
+                         f"of {os.cpu_count()} logical CPUs, {cpu_model()})"}
+        log(f"bench: cpu oracle {ncpu / dt:.4g} rays/s on {nth} threads (median of {times})")
 
     serving = None
     if args.serve_seconds > 0:
