@@ -129,7 +129,9 @@ extern "C" int mcrt_batch_create(mcrt_ctx* ctx, int32_t n, const mcrt_request* r
       return fail(ctx, MCRT_ERR_NOMEM, "batch: allocation failed: %s", hipGetErrorString(e));
     }
     e = hipMemcpy(L.dchunks, h.data(), sizeof(mcrt::BatchChunk) * h.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = mcrt::batch_set_lds(mixed, lds);
+    // the attribute is per kernel and process-wide: set it to the most a batch
+    // kernel can use, so that no solve's launch depends on another's setting
+    if (e == hipSuccess) e = mcrt::batch_set_lds(mixed, ctx->lds_per_block - 16);
     if (e != hipSuccess) {
       delete B;
       return fail(ctx, MCRT_ERR_HIP, "batch: set-up failed: %s", hipGetErrorString(e));

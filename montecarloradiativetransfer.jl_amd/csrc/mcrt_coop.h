@@ -20,6 +20,7 @@ struct CoopPlan {
   int wg_per_cu = 0;  // workgroups launched per CU
   int threads = 0;
   size_t lds_bytes = 0;
+  size_t lds_max = 0;          // LDS per workgroup of the device
   void* params_dev = nullptr;  // CoopParams<R> in device memory
   int nslab = 0;               // workgroup partial-bin slabs
   int64_t slab_stride = 0;     // doubles per slab (sum + compensation)

@@ -1758,6 +1758,7 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
   // active slots per workgroup; by default full slots, fewer when the solve
   // has too few rays to give every CU a workgroup
   cp.lds_bytes = lds;
+  cp.lds_max = lds_max;
   const int wg_per_cu = std::max<int>(1, (int)(lds_max / std::max<size_t>(lds, 1)));
   const int per_cu = shape.workgroups_per_cu > 0 ? std::min(shape.workgroups_per_cu, 4)
                                                  : std::min(wg_per_cu, 2);
@@ -1801,8 +1802,10 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
 template <typename R, int GEO, bool MIE, bool CB>
 hipError_t launch_t(const CoopPlan& cp, hipStream_t s) {
   auto k = coop_kernel<R, GEO, MIE, CB>;
+  // the attribute is per kernel and process-wide: the device maximum, so that
+  // plans of other contexts and threads never see a smaller value set here
   hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)cp.lds_bytes);
+                                     (int)cp.lds_max);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k, dim3(cp.blocks), dim3(cp.threads), cp.lds_bytes, s,
                      (const CoopParams<R>*)cp.params_dev);
