@@ -307,6 +307,22 @@ enum {
 };
 int mcrt_plan_eval(mcrt_plan* plan, int32_t what, int64_t n, const double* in, double* out);
 
+/* simulate with no device allocation: no reference counterpart; it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * outputs and normalisation as mcrt_simulate, for MCRT_PRECISION_FP64 and
+ * MCRT_PRECISION_MIXED.  All device state of the solve (parameters, bin
+ * constants, tables, batch slabs, ray-slot state) lives in a fixed 96 MiB
+ * device array held by each device the library runs on until the process
+ * ends; a solve that needs more is refused with MCRT_ERR_NOMEM and the bytes
+ * it needs in mcrt_last_error.  The solve runs as a one-request batch
+ * (mcrt_batch_*) and equals it up to the order of FP64 additions.  Blocking.
+ * Calls that overlap on one device share the array and race: one call per
+ * device at a time.  stats->kernel_ms is the host-timed launch sequence. */
+int mcrt_simulate_static(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geometry* geom,
+                         const double I0[4], uint64_t seed, int32_t precision, double A[5],
+                         double* Irt, double* Icb, mcrt_stats* stats);
+
 /* ---- batched solves ------------------------------------------------------- */
 
 /* One solve of a batch: the arguments of simulate plus its stream position. */

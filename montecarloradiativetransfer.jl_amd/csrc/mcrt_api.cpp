@@ -151,6 +151,9 @@ extern "C" int mcrt_context_create(int device, mcrt_ctx** out) {
   }
   c->cu_count = prop.multiProcessorCount;
   c->lds_per_block = prop.sharedMemPerBlock;  // 160 KiB on gfx950
+  // the no-allocation solve's scratch address on this device; a failure is
+  // kept and reported by mcrt_simulate_static, the other calls do not need it
+  (void)mcrt::static_scratch_init(device);
   c->tables = mcrt::table_cache_create(kContextTableBytes);
   *out = c;
   return MCRT_OK;

@@ -106,6 +106,10 @@ hipError_t batch_set_lds(bool mixed, size_t lds_bytes);
 hipError_t batch_launch(bool mixed, const BatchChunk* dchunks, int nchunks, unsigned int* dnext,
                         void* slot_mem, int blocks, size_t lds_bytes, hipStream_t stream);
 
+// Find the address of the no-allocation solve's scratch array on `device`
+// (current); mcrt_context_create calls it once per context.
+hipError_t static_scratch_init(int device);
+
 // Validate, stage tables and parameters in device memory.  Returns an MCRT_*
 // status and writes a message to *err on failure.
 int coop_create(CoopPlan& cp, hipStream_t stream, int cu_count, size_t lds_max, const mcrt_config* cfg,

@@ -199,6 +199,8 @@ SIGNATURES = {
                                           C.POINTER(mcrt_launch_shape), C.POINTER(_vp)]),
     "mcrt_plan_shape": (C.c_int, [_vp, C.POINTER(mcrt_launch_shape), C.POINTER(C.c_int32)]),
     "mcrt_plan_eval": (C.c_int, [_vp, C.c_int32, C.c_int64, _dp, _dp]),
+    "mcrt_simulate_static": (C.c_int, [_vp, C.POINTER(mcrt_config), C.POINTER(mcrt_geometry), _dp,
+                                       C.c_uint64, C.c_int32, _dp, _dp, _dp, C.POINTER(mcrt_stats)]),
     "mcrt_batch_create": (C.c_int, [_vp, C.c_int32, C.POINTER(mcrt_request), C.POINTER(_vp)]),
     "mcrt_batch_launch": (C.c_int, [_vp]),
     "mcrt_batch_wait": (C.c_int, [_vp]),
