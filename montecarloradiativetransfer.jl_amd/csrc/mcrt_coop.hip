@@ -32,6 +32,7 @@
 //
 // R is the transport / bin-evaluation type (double: FP64 form, float: mixed).
 #include <algorithm>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <cmath>
@@ -50,13 +51,14 @@ constexpr int kCbRec = 40;  // R words of a CB event record
 constexpr int kCbDx = 4;    // doubles of a CB event record (x_a - x_b, τab)
 constexpr int kBinW = 11;     // R words of a bin used by the evaluation
 constexpr int kBinStride = 12;  // R words per bin in the bin tables
+constexpr int kAcosTerms = 7;   // coefficients of the FP32 arccos polynomial (degree 6)
 
 // Per-precision launch shape: ray slots per workgroup and RT bins per
 // register chunk.
 template <typename R> struct Shape;
 template <> struct Shape<float> {
   static constexpr int kSlots = 256;
-  static constexpr int kChunk = 4;
+  static constexpr int kChunk = 8;
   static constexpr bool kStageS = true;  // S table in LDS
   // ρ² below which a bin counts as aligned with the event direction (the
   // polar branch): FP32 rounding leaves ρ² ≈ 1e-13 for exactly (anti)parallel
@@ -75,10 +77,11 @@ template <typename R> struct CoopParams {
   int64_t maxscat;
   int slots;      // active ray slots per workgroup (multiple of 64, ≤ Shape<R>::kSlots)
   int rt_cut;     // 0: no peel-off distance can exceed τ₀ (sphere with 2τ < τ₀)
+  int cb_cut;     // 0: no CB distance can exceed τ₀ (sphere with 4τ < τ₀)
   R min_int;
   R tau0;
   R tau0k;      // tau0 · ExpK<R>::k
-  R acs[8];     // FP32 arccos coefficients times (Nθ_table - 1)/π
+  R acs[kAcosTerms];  // FP32 arccos coefficients times (Nθ_table - 1)/π, highest degree first
   R pis;        // π · (Nθ_table - 1)/π = Nθ_table - 1, the last node
   int compute_cb;
   int geo_kind;  // 0 plane, 1 sphere
@@ -159,8 +162,9 @@ MCRT_DEV void kahan(double* s, double* c, double x) {
 }
 
 // Table coordinate sn = acos(μ)·(Nθ-1)/π of a phase / amplitude lookup
-// (mie.jl:53-55).  FP32: Abramowitz & Stegun 4.4.46 (Hastings),
-// acos(a) = √(1-a)·p(a) on [0, 1] with |error| ≤ 2e-8, its coefficients
+// (mie.jl:53-55).  FP32: acos(a) = √(1-a)·p(a) on [0, 1] with p the degree-6
+// weighted minimax polynomial of csrc/fit_acos.py (Remez exchange, |acos error|
+// ≤ 8.8e-8 rad, below the FP32 rounding of θ and of μ itself), its coefficients
 // pre-multiplied by (Nθ-1)/π, and acos(-a) = π - acos(a).  FP64: the library acos.
 template <typename R> MCRT_DEV R theta_index(const CoopParams<R>& p, R mu);
 template <> MCRT_DEV double theta_index(const CoopParams<double>& p, double mu) {
@@ -170,21 +174,22 @@ template <> MCRT_DEV float theta_index(const CoopParams<float>& p, float mu) {
   const float a = __builtin_fabsf(mu);
   float c = p.acs[0];
 #pragma unroll
-  for (int k = 1; k < 8; ++k) c = __builtin_fmaf(c, a, p.acs[k]);
+  for (int k = 1; k < kAcosTerms; ++k) c = __builtin_fmaf(c, a, p.acs[k]);
   // 1 - a clamped to [0, 1] (|μ| can exceed 1 by rounding): the clamp bit of the subtraction
   const float r = __builtin_amdgcn_sqrtf(__builtin_amdgcn_fmed3f(1.0f - a, 0.0f, 1.0f)) * c;
   return mu < 0.0f ? p.pis - r : r;
 }
 
-// √max(x, 0): one max and the hardware square root in FP32.
-MCRT_DEV float sqrt_pos(float x) { return __builtin_amdgcn_sqrtf(__builtin_fmaxf(x, 0.0f)); }
-MCRT_DEV double sqrt_pos(double x) { return ::sqrt(x < 0.0 ? 0.0 : x); }
+MCRT_DEV float fmax_r(float a, float b) { return __builtin_fmaxf(a, b); }
+MCRT_DEV double fmax_r(double a, double b) { return a > b ? a : b; }
 
-// Split sn into the node index i0 ≤ imax and the fraction f = sn - i0.
-MCRT_DEV int node_of(float sn, float imax, float& f) {
-  const float s = __builtin_fminf(sn, imax);
-  f = __builtin_amdgcn_fractf(s);
-  return (int)s;
+// Split sn into the node index i0 ≤ imax and the fraction f = sn - i0.  In
+// FP32, theta_index already lies in [0, imax] (its polynomial is positive, so
+// r ≥ 0 and sn = r ≤ imax/2 or sn = imax - r), and at sn = imax the padded zero
+// node takes the zero weight: no clamp is needed.
+MCRT_DEV int node_of(float sn, float, float& f) {
+  f = __builtin_amdgcn_fractf(sn);
+  return (int)sn;
 }
 MCRT_DEV int node_of(double sn, double imax, double& f) {
   const double s = sn < imax ? sn : imax;
@@ -348,15 +353,18 @@ MCRT_DEV void peel_I(const CoopParams<R>& p, const R* F, const R* I, const R* J,
 }
 
 // update_E (utils.jl:170-192) on the scattering plane q: E ← Rψ·S(μ)·Rϕₖ·E,
-// returned times ρ² (multiply by q.inv).
+// returned times ρ²·sc (sc = q.inv gives update_E itself); the real factor sc
+// (e.g. an attenuation) rides on the two coefficients of the ψ rotation.
 template <typename R, bool MIE>
-MCRT_DEV void peel_E(const CoopParams<R>& p, const R* S, const Plane<R>& q, Cx<R>& E1, Cx<R>& E2) {
+MCRT_DEV void peel_E(const CoopParams<R>& p, const R* S, const Plane<R>& q, R sc, Cx<R>& E1,
+                     Cx<R>& E2) {
   Cx<R> S11, S22;
   amp_tab<R, MIE>(p, S, q.mu, S11, S22);
   const Cx<R> f1 = S11 * (E1 * q.rx + E2 * q.ry);
   const Cx<R> f2 = S22 * (E2 * q.rx + E1 * -q.ry);
-  E1 = f1 * q.pn + f2 * -q.w;
-  E2 = f1 * q.w + f2 * q.pn;
+  const R pn = q.pn * sc, w = q.w * sc;
+  E1 = f1 * pn + f2 * -w;
+  E2 = f1 * w + f2 * pn;
 }
 
 // Attenuation scale: positions and path lengths of the RT records are stored
@@ -371,15 +379,26 @@ template <> struct ExpK<double> {
   static MCRT_DEV double exp_neg(double t) { return ::exp(-t); }
 };
 
-// cos/sin of a large FP64 phase: exact FP64 reduction to revolutions, then the
-// R-precision evaluation of the reduced angle.
+// cos/sin of the large FP64 CB phase, given in the unit PhaseUnit<R> (radians
+// for FP64; revolutions for FP32, so that an exact FP64 reduction to
+// [-1/2, 1/2] revolutions precedes the FP32 evaluation).
+template <typename R> struct PhaseUnit;
+template <> struct PhaseUnit<double> { static constexpr double per_radian = 1.0; };
+template <> struct PhaseUnit<float> { static constexpr double per_radian = 0.15915494309189535; };
 MCRT_DEV void phase_cs(double phi, double& c, double& s) { sincos(phi, &s, &c); }
-MCRT_DEV void phase_cs(double phi, float& c, float& s) {
-  double rev = phi * 0.15915494309189535;  // 1/(2π)
+MCRT_DEV void phase_cs(double rev, float& c, float& s) {
   float f = (float)(rev - rint(rev));      // in [-1/2, 1/2] revolutions
   s = __builtin_amdgcn_sinf(f);            // v_sin_f32 takes revolutions
   c = __builtin_amdgcn_cosf(f);
 }
+
+// The kind word of a CB record (0 none, 1 n = 1, 2 n > 1) as integer bits in
+// an R word, read without a conversion.
+template <typename R> MCRT_DEV R kind_word(int k);
+template <> MCRT_DEV float kind_word<float>(int k) { return __int_as_float(k); }
+template <> MCRT_DEV double kind_word<double>(int k) { return __longlong_as_double((long long)k); }
+MCRT_DEV int kind_of(float w) { return __float_as_int(w); }
+MCRT_DEV int kind_of(double w) { return (int)__double_as_longlong(w); }
 
 MCRT_DEV void lmap_normalise(Lmap<double>& Q, double& lq) {
   double m = fmax(fmax(fmax(fabs(Q.a.re), fabs(Q.a.im)), fmax(fabs(Q.b.re), fabs(Q.b.im))),
@@ -498,7 +517,7 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
           t = (bk[i][2] > R(0) ? rec[16] : rec[16] + rec[17]) * -bk[i][9];
         } else {         // sphere.jl:69-73
           const R kx = bk[i][0] * rec[14] + bk[i][1] * rec[15] + bk[i][2] * rec[16];
-          t = sqrt_pos(kx * kx + rec[17]) - kx;
+          t = M<R>::sqrt(kx * kx + rec[17]) - kx;  // rec[17] ≥ 0: the radicand is too
         }
         // the τ₀ cut-off (cpu.jl:20); CUT = false when no distance can exceed it
         const R w = (!CUT || t <= tau0k) ? bk[i][8] * ExpK<R>::exp_neg(t) : R(0);
@@ -541,6 +560,16 @@ MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F, const R* rtrec, R* e
     const int fw = p.range_kind[rg] == 1 ? 18 : 19;  // record word of the near flag
     const int slot2 = p.range_kind[rg] == 1 ? 0 : 1;
     int c0 = 0;
+    // with chunks of 8, a remainder of one bin joins the last chunk (a pass of
+    // 9: e.g. 17 bins per thread at 180 × 48 run as 9 + 8, and 9 as one pass),
+    // since every pass pays the per-event overhead (record reads, reduction)
+    if constexpr (CH == 8) {
+      if (per % 8 == 1 && per >= 9) {
+        rt_pass<9, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow, nb,
+                                     c0, fw, slot2);
+        c0 += 9;
+      }
+    }
     for (; c0 + CH <= per; c0 += CH)
       rt_pass<CH, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb,
                                             nrow, nb, c0, fw, slot2);
@@ -569,15 +598,19 @@ MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F, const R* rtrec, R* e
 // every group with atomics.
 // CB record: [0..7] the scaled forward and reversed fields (n > 1) or I and
 // [32..33] J (n = 1); [8..15] frame matrix of the event direction, [16..23]
-// of -k₂; [24..26] x, [27..29] x₁; [30], [31] the layer τ at x and x₁ (plane)
-// or τ² - x·x and τ² - x₁·x₁ (sphere); [39] kind (0 none, 1 n = 1, 2 n > 1).
-template <typename R, int GEO, bool MIE, bool ATOMIC>
+// of -k₂; [24..26] K·x, [27..29] K·x₁; [30], [31] K·τ of the layer at x and x₁
+// (plane) or K²·(τ² - x·x) and K²·(τ² - x₁·x₁), both ≥ 0 (sphere); [39] kind
+// (0 none, 1 n = 1, 2 n > 1); K = ExpK<R>::k, so that distances come out
+// scaled for exp2 as in the peel-off.  CUT = false when no CB distance can
+// exceed τ₀ (a sphere with 4τ < τ₀: τ_a ≤ 2τ and |τ_b + τ_ab| ≤ 4τ), where both
+// cut-off tests (cpu.jl:97, 104) always pass.
+template <typename R, int GEO, bool MIE, bool ATOMIC, bool CUT>
 MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* cbrec,
                       const double* cbdx, double* slab, int E, int tid) {
   const int nb = p.Ntb * p.Npb;
   // split only iterations with enough events to share (uniform over the workgroup)
   const int G = E >= 8 * p.cb_groups ? p.cb_groups : 1;
-  const R tau0 = p.tau0;
+  const R tau0k = p.tau0k;
   const int total = nb * G;
   // a work item's bin constants; the next item's are loaded before the flush
   // of the current one (loads issued after its atomics would wait for them)
@@ -600,61 +633,65 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
     R acc[4] = {R(0), R(0), R(0), R(0)};
     for (int e = grp; e < E; e += G) {
       const R* cr = cbrec + kCbRec * e;
-      const int kind = (int)cr[39];
+      const int kind = kind_of(cr[39]);
       if (kind == 0) continue;
       const V3<R> xa = {cr[24], cr[25], cr[26]};
-      R ta;
+      R ta;  // K·τ_a
       if (GEO == 0) ta = kfz > R(0) ? -xa.z * ikz : (xa.z + cr[30]) * -ikz;
       else {
         R kx = kfx * xa.x + kfy * xa.y + kfz * xa.z;
-        ta = sqrt_pos(kx * kx + cr[30]) - kx;
+        ta = M<R>::sqrt(kx * kx + cr[30]) - kx;
       }
-      if (!(ta <= tau0)) continue;
+      if (CUT && !(ta <= tau0k)) continue;
       const Plane<R> qa = scat_plane(cr + 8, kfx, kfy, kfz, b[3], b[4], b[5], b[6], b[7]);
-      R o[4];
+      // the bin's Stokes vector o before the final rotation, and the factors
+      // s0 (I, V) and s12 (Q, U) that scale it into the accumulation
+      R o[4], s0, s12;
       if (kind == 1) {  // n = 1 (cpu.jl:131-158)
         peel_I<R, MIE>(p, F, cr, cr + 32, qa, o);
-        const R na = M<R>::exp(-ta);
-        const R ni = na * qa.inv;
-        o[0] = o[0] * na; o[1] = o[1] * ni; o[2] = o[2] * ni; o[3] = o[3] * na;
+        s0 = ExpK<R>::exp_neg(ta);
+        s12 = s0 * qa.inv;
       } else {          // n > 1 (cpu.jl:83-127)
         const V3<R> xb = {cr[27], cr[28], cr[29]};
         Cx<R> a1 = {cr[0], cr[1]}, a2 = {cr[2], cr[3]};
         Cx<R> b1 = {cr[4], cr[5]}, b2 = {cr[6], cr[7]};
         const Plane<R> qb = scat_plane(cr + 16, kfx, kfy, kfz, b[3], b[4], b[5], b[6], b[7]);
-        peel_E<R, MIE>(p, S, qa, a1, a2);
-        peel_E<R, MIE>(p, S, qb, b1, b2);
         const double* dx = cbdx + kCbDx * e;
-        const double dphi = p.kl1 * (kikx * dx[0] + kiky * dx[1] + kikz * dx[2]);
-        const R tab = (R)dx[3];
-        const R na = M<R>::exp(R(-0.5) * ta) * qa.inv;
-        a1 = a1 * na; a2 = a2 * na;
-        R Ia[4];
-        I_from_E(a1, a2, Ia);
+        const R tab = (R)dx[3];  // K·τ_ab
         R tb;
         if (GEO == 0) tb = kfz > R(0) ? -xb.z * ikz : (xb.z + cr[31]) * -ikz;
         else {
           R kx = kfx * xb.x + kfy * xb.y + kfz * xb.z;
-          tb = sqrt_pos(kx * kx + cr[31]) - kx;
+          tb = M<R>::sqrt(kx * kx + cr[31]) - kx;
         }
-        const bool bok = M<R>::fabs(tb + tab) <= tau0;
-        const R nb_ = bok ? M<R>::exp(R(-0.5) * (tb + tab)) * qb.inv : R(0);
-        b1 = b1 * nb_; b2 = b2 * nb_;
-        R Ib[4];
-        I_from_E(b1, b2, Ib);
+        const bool bok = !CUT || M<R>::fabs(tb + tab) <= tau0k;
+        // the attenuations e^{-τ/2} and the 1/ρ² of each plane ride on peel_E's ψ rotation
+        const R na = ExpK<R>::exp_neg(R(0.5) * ta) * qa.inv;
+        const R nb_ = bok ? ExpK<R>::exp_neg(R(0.5) * (tb + tab)) * qb.inv : R(0);
+        peel_E<R, MIE>(p, S, qa, na, a1, a2);
+        peel_E<R, MIE>(p, S, qb, nb_, b1, b2);
+        // the phase kl·(k_i + k_f)·(x_a - x_b) (cpu.jl:94), kl and the unit folded
+        // into the per-bin k_i + k_f
+        const double dphi = kikx * dx[0] + kiky * dx[1] + kikz * dx[2];
+        // only the intensities of the two fields are needed (cpu.jl:100, 108)
+        const R Ia0 = abs2(a1) + abs2(a2);
+        const R Ib0 = abs2(b1) + abs2(b2);
         R cph, sph_;
         phase_cs(dphi, cph, sph_);
         Cx<R> ep = {cph, sph_};
         Cx<R> e1 = a1 * ep + b1, e2 = a2 * ep + b2;
         I_from_E(e1, e2, o);
-        const R den = Ia[0] + Ib[0];
-        const R rr = den > R(0) ? M<R>::div(Ia[0], den) : R(0);  // 0 where both fields vanish
-#pragma unroll
-        for (int c = 0; c < 4; ++c) o[c] = o[c] * rr;
+        // Ia/(Ia + Ib) (cpu.jl:114); 0 where both fields vanish (0·finite)
+        s0 = M<R>::div(Ia0, fmax_r(Ia0 + Ib0, std::numeric_limits<R>::min()));
+        s12 = s0;
       }
+      // the final rotation (cpu.jl:118-124) is linear, so it acts before the
+      // scale factors, which then fold into the accumulating FMAs
       rotate(o, b[8], b[9]);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) acc[c] += o[c];
+      acc[0] = fma_r(o[0], s0, acc[0]);
+      acc[1] = fma_r(o[1], s12, acc[1]);
+      acc[2] = fma_r(o[2], s12, acc[2]);
+      acc[3] = fma_r(o[3], s0, acc[3]);
     }
     const int nit = it + kThreads;
     Item nxt;
@@ -973,7 +1010,7 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
           const R K = ExpK<R>::k;
           rec[14] = K * s.x.x; rec[15] = K * s.x.y; rec[16] = K * s.x.z;
           if (GEO == 0) rec[17] = p.isinf[s.idx] ? R(INFINITY) : K * p.tau[s.idx];
-          else rec[17] = K * K * (p.tau[0] * p.tau[0] - dot(s.x, s.x));
+          else rec[17] = K * K * fmax_r(p.tau[0] * p.tau[0] - dot(s.x, s.x), R(0));
         }
         rec[18] = R(near_up);
         rec[19] = R(near_lo);
@@ -1017,20 +1054,22 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
             frame_matrix(Fr<R>{(R)kas.ct, (R)kas.st, (R)kas.cp, (R)kas.sp}, ma_);
             frame_matrix(Fr<R>{(R)s.kbs.ct, (R)s.kbs.st, (R)s.kbs.cp, (R)s.kbs.sp}, mb_);
             for (int c = 0; c < 8; ++c) { cr[8 + c] = ma_[c]; cr[16 + c] = mb_[c]; }
-            cr[24] = s.x.x; cr[25] = s.x.y; cr[26] = s.x.z;
-            cr[27] = (R)s.x1d.x; cr[28] = (R)s.x1d.y; cr[29] = (R)s.x1d.z;
+            // positions and path lengths scaled by K for exp2 (see cb_pass)
+            const R K = ExpK<R>::k;
+            const V3<R> xb = {(R)s.x1d.x, (R)s.x1d.y, (R)s.x1d.z};
+            cr[24] = K * s.x.x; cr[25] = K * s.x.y; cr[26] = K * s.x.z;
+            cr[27] = K * xb.x; cr[28] = K * xb.y; cr[29] = K * xb.z;
             if (GEO == 0) {
-              cr[30] = p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx];
-              cr[31] = p.isinf[s.idx1] ? R(INFINITY) : p.tau[s.idx1];
-            } else {  // τ² - x·x of the sphere's distance (sphere.jl:69-73)
-              const V3<R> xb = {cr[27], cr[28], cr[29]};
-              cr[30] = p.tau[0] * p.tau[0] - dot(s.x, s.x);
-              cr[31] = p.tau[0] * p.tau[0] - dot(xb, xb);
+              cr[30] = p.isinf[s.idx] ? R(INFINITY) : K * p.tau[s.idx];
+              cr[31] = p.isinf[s.idx1] ? R(INFINITY) : K * p.tau[s.idx1];
+            } else {  // τ² - x·x of the sphere's distance (sphere.jl:69-73), ≥ 0
+              cr[30] = K * K * fmax_r(p.tau[0] * p.tau[0] - dot(s.x, s.x), R(0));
+              cr[31] = K * K * fmax_r(p.tau[0] * p.tau[0] - dot(xb, xb), R(0));
             }
             double* dx = cbdx + kCbDx * e_me;
             dx[0] = xa.x - s.x1d.x; dx[1] = xa.y - s.x1d.y; dx[2] = xa.z - s.x1d.z;
-            dx[3] = ta - tb;  // τab
-            cr[39] = R(2);
+            dx[3] = (ta - tb) * (double)ExpK<R>::k;  // K·τab
+            cr[39] = kind_word<R>(2);
           } else if (cb_on) {
             // collect_cb n = 1 (cpu.jl:131-158)
             R ma_[8];
@@ -1038,12 +1077,13 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
             for (int c = 0; c < 8; ++c) cr[8 + c] = ma_[c];
             cr[0] = s.I[0]; cr[1] = s.I[1]; cr[2] = s.I[2]; cr[3] = s.I[3];
             cr[32] = R(2) * s.I[1]; cr[33] = R(-2) * s.I[2];
-            cr[24] = s.x.x; cr[25] = s.x.y; cr[26] = s.x.z;
-            cr[30] = GEO == 0 ? (p.isinf[s.idx] ? R(INFINITY) : p.tau[s.idx])
-                              : p.tau[0] * p.tau[0] - dot(s.x, s.x);
-            cr[39] = R(1);
+            const R K = ExpK<R>::k;
+            cr[24] = K * s.x.x; cr[25] = K * s.x.y; cr[26] = K * s.x.z;
+            cr[30] = GEO == 0 ? (p.isinf[s.idx] ? R(INFINITY) : K * p.tau[s.idx])
+                              : K * K * fmax_r(p.tau[0] * p.tau[0] - dot(s.x, s.x), R(0));
+            cr[39] = kind_word<R>(1);
           } else {
-            cr[39] = R(0);
+            cr[39] = kind_word<R>(0);
           }
         }
       }
@@ -1061,7 +1101,10 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
       rt_ranges<R, GEO, MIE, true>(p, ldsF, rtrec, esc, slab, E, tid, lane, wave);
 
     // ------------------------------------------ CB bins × events (cpu.jl:38-160)
-    if (CB && p.compute_cb) cb_pass<R, GEO, MIE, ATOMIC>(p, ldsF, Sbins, cbrec, cbdx, slab, E, tid);
+    if (CB && p.compute_cb) {
+      if (GEO == 1 && !p.cb_cut) cb_pass<R, GEO, MIE, ATOMIC, false>(p, ldsF, Sbins, cbrec, cbdx, slab, E, tid);
+      else cb_pass<R, GEO, MIE, ATOMIC, true>(p, ldsF, Sbins, cbrec, cbdx, slab, E, tid);
+    }
     __syncthreads();
 
     // ---------------------- weight update and termination (cpu.jl:263-272)
@@ -1220,8 +1263,8 @@ __global__ void eval_kernel(const CoopParams<R>* __restrict__ pp, int what, int6
         r[2] = r[2] * q.inv;
       } else {
         Cx<R> E1 = {(R)a[4], (R)a[5]}, E2 = {(R)a[6], (R)a[7]};
-        peel_E<R, MIE>(p, p.sca.S, q, E1, E2);
-        r[0] = E1.re * q.inv; r[1] = E1.im * q.inv; r[2] = E2.re * q.inv; r[3] = E2.im * q.inv;
+        peel_E<R, MIE>(p, p.sca.S, q, q.inv, E1, E2);
+        r[0] = E1.re; r[1] = E1.im; r[2] = E2.re; r[3] = E2.im;
       }
       break;
     }
@@ -1623,15 +1666,19 @@ int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_co
   if (rc) return rc;
   {
     const double sc = mie ? (s0->n_theta - 1) / 3.141592653589793 : 0.0;
-    // Abramowitz & Stegun 4.4.46: acos(a) = √(1-a)·Σ cₖ aᵏ on [0, 1]
-    static const double c[8] = {-0.0012624911, 0.0066700901, -0.0170881256, 0.0308918810,
-                                -0.0501743046, 0.0889789874, -0.2145988016, 1.5707963050};
+    // acos(a) = √(1-a)·Σ cₖ aᵏ on [0, 1]: the degree-6 minimax fit printed by
+    // csrc/fit_acos.py, highest degree first
+    static const double c[kAcosTerms] = {0.0026117211109719563, -0.012003396496319869,
+                                         0.027762914927838391,  -0.049197437593977882,
+                                         0.088835885904433951,  -0.21459108867891987,
+                                         1.5707962388530227};
     P.scale = (R)sc;
-    for (int k = 0; k < 8; ++k) P.acs[k] = (R)(c[k] * sc);
+    for (int k = 0; k < kAcosTerms; ++k) P.acs[k] = (R)(c[k] * sc);
     P.pis = mie ? (R)(s0->n_theta - 1) : R(0);  // π·sc
     P.tau0k = (R)(cfg->tau0 * (double)ExpK<R>::k);
     // inside a sphere every distance to the boundary is at most the diameter
     P.rt_cut = !(g->kind == MCRT_GEOMETRY_SPHERE && 2.0 * g->tau * (1.0 + 1e-6) < cfg->tau0);
+    P.cb_cut = !(g->kind == MCRT_GEOMETRY_SPHERE && 4.0 * g->tau * (1.0 + 1e-6) < cfg->tau0);
   }
   const int Nt = (int)cfg->n_theta, Np = (int)cfg->n_phi;
   const int Ntb = cb ? (int)cfg->n_theta_b : 0, Npb = cb ? (int)cfg->n_phi_b : 0;
@@ -1693,7 +1740,10 @@ int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_co
       b[8] = (R)(cps * cps - sps * sps);
       b[9] = (R)(2.0 * cps * sps);
       b[10] = (R)(1.0 / kf[2]);
-      kik[4 * lb] = k0[0] + kf[0]; kik[4 * lb + 1] = k0[1] + kf[1]; kik[4 * lb + 2] = k0[2] + kf[2];
+      // kl·(k_i + k_f) in the unit of phase_cs (cpu.jl:94)
+      const double ku = gd.kl1 * PhaseUnit<R>::per_radian;
+      kik[4 * lb] = ku * (k0[0] + kf[0]); kik[4 * lb + 1] = ku * (k0[1] + kf[1]);
+      kik[4 * lb + 2] = ku * (k0[2] + kf[2]);
       kik[4 * lb + 3] = 0;
     }
   auto up = [&](const void* h, size_t bytes) -> void* {
