@@ -224,7 +224,15 @@ def main() -> int:
                    "rate_per_s": rate, "seconds": args.serve_seconds,
                    "requests": rep["requests"], "completed": rep["completed"],
                    "rays_per_s": rep["rays_per_s"], "p50_ms": rep.get("p50_ms"),
-                   "p99_ms": rep.get("p99_ms"), "launches": rep["server"]["launches"],
+                   "p99_ms": rep.get("p99_ms"),
+                   # per request: submit → admission (queue), admission → device start of
+                   # the first window running one of its chunks (window), then to done (device)
+                   "queue_ms": rep["breakdown"]["queue_ms"],
+                   "window_wait_ms": rep["breakdown"]["wait_ms"],
+                   "device_ms": rep["breakdown"]["device_ms"],
+                   "launches": rep["server"]["launches"],
+                   "preemptions": rep["server"]["preemptions"],
+                   "resumed_chunks": rep["server"]["resumed"],
                    "table_hits": rep["server"]["table_hits"],
                    "table_misses": rep["server"]["table_misses"]}
         log(f"bench: serving {rep['completed']}/{rep['requests']} requests, "

@@ -141,6 +141,10 @@ typedef struct mcrt_stats {
                             slabs of a plan, batch slabs of a batched or served
                             solve (0: the plain form, which has none) */
   uint64_t chunks;       /* work chunks of a batched or served solve (0: plan) */
+  double queue_ms;       /* served: submit to admission (0 elsewhere) */
+  double wait_ms;        /* served: admission to the device start of the first
+                            launch that ran one of its chunks (0 elsewhere);
+                            kernel_ms is then that start to completion */
 } mcrt_stats;
 
 typedef struct mcrt_ctx mcrt_ctx;
@@ -338,8 +342,8 @@ typedef struct mcrt_request {
                            chunks when N >= 2.  Batches: ceil(N/64) rays within
                            512..8192 (throughput).  The server: about 4e7 bin
                            evaluations per chunk from the solve's bins and a
-                           model of its events per ray, 64..8192 rays (a launch
-                           ends no sooner than its longest chunk: latency). */
+                           model of its events per ray, 64..8192 rays (a solve
+                           is done no sooner than its longest chunk: latency). */
 } mcrt_request;
 
 typedef struct mcrt_batch mcrt_batch;
@@ -380,14 +384,16 @@ enum {
 
 typedef struct mcrt_server_stats {
   uint64_t submitted, completed, cancelled, failed;
-  uint64_t launches;          /* persistent launches issued */
-  uint64_t chunks;            /* chunks in those launches */
+  uint64_t launches;          /* persistent launches (windows) issued */
+  uint64_t chunks;            /* tasks in those launches (chunks started or resumed) */
   uint64_t rays;              /* rays of the completed solves */
   uint64_t queued, running;   /* tickets now waiting / admitted */
   int64_t device_bytes_in_use, device_budget;
   int64_t table_cache_bytes;
   uint64_t table_hits, table_misses;
-  double busy_ms;             /* wall time with at least one launch in flight */
+  double busy_ms;             /* wall time with a launch or a read-back in flight */
+  uint64_t preemptions;       /* windows asked to stop early for waiting work */
+  uint64_t resumed;           /* chunks resumed from a stopped window */
 } mcrt_server_stats;
 
 typedef struct mcrt_server mcrt_server;
@@ -395,12 +401,16 @@ typedef struct mcrt_server mcrt_server;
 /* A request loop on its own host thread and two HIP streams of the context's
  * device.  Submitted solves wait until their device memory (bins, batch
  * slabs, parameters) fits device_budget_bytes (<= 0: half of the free device
- * memory); Mie tables are kept resident across requests in a cache keyed by
- * their content (table_cache_bytes, <= 0: 1 GiB).  Admitted solves of one
- * precision share each persistent launch (continuous batching: a solve that
- * arrives while others run joins the next launch, queued on the device behind
- * the current one), and every solve's result is the same alone and in the mix
- * up to the order of FP64 additions.  No reference counterpart. */
+ * memory) and fewer than 1024 are admitted; Mie tables are kept resident
+ * across requests in a cache keyed by their content (table_cache_bytes,
+ * <= 0: 1 GiB).  Admitted solves of one precision share persistent launch
+ * windows, one at a time: while other solves wait, the running window is
+ * asked to stop after a time slice (at once when part of the device is idle),
+ * its workgroups finish the iteration they are in and keep their chunks' rays
+ * in flight, and the next window resumes those chunks round-robin with the
+ * waiting ones.  Every solve's result is the same alone and in the mix, and
+ * wherever the windows were cut, up to the order of FP64 additions.  No
+ * reference counterpart. */
 int mcrt_server_create(mcrt_ctx* ctx, int64_t device_budget_bytes, int64_t table_cache_bytes,
                        mcrt_server** out);
 /* Queue a solve (the request and what it points to are copied). */
@@ -416,9 +426,12 @@ int mcrt_wait(mcrt_server* srv, uint64_t ticket, double timeout_ms, int32_t* sta
 int mcrt_result(mcrt_server* srv, uint64_t ticket, double A[5], double* Irt, double* Icb,
                 double A_sigma[5], double* Irt_sigma, double* Icb_sigma, mcrt_stats* stats,
                 double* latency_ms);
-/* Stop a queued or running ticket (no effect once its last chunk launched). */
+/* Stop a queued or running ticket (no effect once its last chunk finished). */
 int mcrt_cancel(mcrt_server* srv, uint64_t ticket);
 int mcrt_server_stats_get(mcrt_server* srv, mcrt_server_stats* out);
+/* How long a window keeps the device while other work waits (default 8 ms;
+ * 0: stop it as soon as work waits).  No reference counterpart. */
+int mcrt_server_set_slice(mcrt_server* srv, double slice_ms);
 /* Cancel what is left, wait for the device, free everything. */
 void mcrt_server_destroy(mcrt_server* srv);
 

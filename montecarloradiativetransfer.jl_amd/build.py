@@ -18,8 +18,8 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libmcrt_hip.so"
-SOURCES = ["mcrt_tables.cpp", "mcrt_api.cpp", "mcrt_batch.cpp", "mcrt_serve.cpp", "mcrt_plain.hip",
-           "mcrt_coop.hip", "mcrt_static.hip"]
+SOURCES = ["mcrt_tables.cpp", "mcrt_api.cpp", "mcrt_batch.cpp", "mcrt_sched.cpp", "mcrt_serve.cpp",
+           "mcrt_plain.hip", "mcrt_coop.hip", "mcrt_static.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

@@ -41,11 +41,17 @@ struct CoopPlan {
 using DevAlloc = std::function<void*(size_t, hipError_t*)>;
 
 // A ray range [rb, re) of one solve of a batched launch; the sums go to slab
-// (one of the solve's batch slabs) with FP64 atomics.
+// (one of the solve's batch slabs) with FP64 atomics.  cont: the chunk's own
+// ray-slot block (NULL: the workgroup's block in the launch's slot memory);
+// resume: cont holds rays in flight from a launch that stopped the chunk, and
+// rb is the first ray not yet started.
 struct BatchChunk {
   const void* params;  // the solve's parameter block (device)
   int64_t rb, re;
   double* slab;
+  void* cont = nullptr;
+  int32_t resume = 0;
+  int32_t pad = 0;
 };
 
 // Device copies of scatterer tables keyed by their content, shared by the
@@ -102,9 +108,18 @@ hipError_t solve_finish(const SolveDev& sd, hipStream_t stream);
 size_t batch_slot_bytes(bool mixed, int blocks);
 hipError_t batch_set_lds(bool mixed, size_t lds_bytes);
 // One persistent launch over nchunks chunks (dnext: a device counter it zeroes
-// first); blocks workgroups, each with its slot state in slot_mem.
+// first, unless zero_next is false and the caller has cleared it); blocks
+// workgroups, each with its slot state in slot_mem (or the chunk's cont).
+// With preempt (host memory the device reads), a workgroup that sees a
+// non-zero flag stops its chunk at the next iteration boundary, after at
+// least one iteration, and claims no further chunk; states[c] then receives
+// chunk c's outcome: -2 finished, or the first ray not yet started (the
+// rays in flight are kept in the chunk's cont), while unclaimed chunks keep
+// what the caller stored there.
 hipError_t batch_launch(bool mixed, const BatchChunk* dchunks, int nchunks, unsigned int* dnext,
-                        void* slot_mem, int blocks, size_t lds_bytes, hipStream_t stream);
+                        void* slot_mem, int blocks, size_t lds_bytes, hipStream_t stream,
+                        int64_t* states = nullptr, const unsigned int* preempt = nullptr,
+                        bool zero_next = true);
 
 // Find the address of the no-allocation solve's scratch array on `device`
 // (current); mcrt_context_create calls it once per context.

@@ -1,19 +1,20 @@
 // mcrt_serve.cpp — the request loop (C ABI mcrt_server_*, mcrt_submit, …).
 //
-// Continuous batching on one device: a host thread admits submitted solves
-// while their device memory (bins, slabs, parameters; Mie tables are shared
-// through a content-keyed cache) fits a budget, and keeps two launch lanes
-// (HIP streams) busy with persistent batch_kernel launches.  Each launch takes
-// a window of chunks round-robin from every admitted solve of one precision,
-// so a solve that arrives while others run joins the next window; with two
-// lanes the next window is queued on the device while the current one runs
-// and its workgroups start as the current one's finish.  A solve is cut into
-// chunks by a rule that depends on the solve alone, and a chunk never shares
-// an iteration with another, so every result is the same alone and in the mix
-// up to the order of FP64 additions.
+// Continuous batching on one device: a host thread runs the scheduler of
+// mcrt_sched.h (admission by device memory, launch windows of chunks taken
+// round-robin from every admitted solve of one precision, preemption of the
+// running window while work waits, resumption of the chunks it stopped) over
+// the HIP backend below: one compute stream for the windows and the
+// reductions that follow them, one set-up stream for clearing the slabs of
+// newly admitted solves.  A solve is cut into chunks by a rule that depends on
+// the solve alone, a chunk never shares an iteration with another, and a
+// stopped chunk resumes exactly where it stopped, so every result is the same
+// alone, in the mix and wherever the windows were cut, up to the order of FP64
+// additions.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -28,9 +29,11 @@
 #include "mcrt_coop.h"
 #include "mcrt_internal.h"
 #include "mcrt_kernels.h"
+#include "mcrt_sched.h"
 
 using mcrt::fail;
 using Clock = std::chrono::steady_clock;
+namespace sched = mcrt::sched;
 
 namespace {
 
@@ -166,6 +169,7 @@ class Pool {
   std::map<size_t, std::vector<void*>> free_;
 };
 
+
 struct Ticket {
   uint64_t id = 0;
   OwnedRequest r;
@@ -173,15 +177,11 @@ struct Ticket {
   int err_code = 0;
   std::string err;
   bool mixed = false;
-  bool cancel = false;
   size_t need = 0;
-  bool admitted = false;  // holds `need` bytes of the budget
   mcrt::SolveDev sd;
   std::vector<std::pair<void*, size_t>> allocs;  // from the server's device pool
-  int64_t next_chunk = 0;
-  int inflight = 0;           // launches still running with chunks of this solve
-  bool finished = false;      // the launch with its reduction and read-back completed
-  bool released = false;      // device memory freed
+  bool released = true;
+  bool admitted = false;  // set by the worker under the server lock
   // read-back of the results: one pinned block (host_pool) holding the sums,
   // the standard errors and the counters
   void* h_block = nullptr;
@@ -189,16 +189,54 @@ struct Ticket {
   double* h_out = nullptr;
   double* h_sig = nullptr;
   unsigned long long* h_cnt = nullptr;
-  Clock::time_point t_submit, t_start, t_done;
+  Clock::time_point t_submit, t_admit, t_start, t_done;
+  bool started = false;
 };
 
-constexpr int kLanes = 2;
-constexpr int kLaneChunks = 1 << 16;  // chunk-list capacity of one launch
-// Admitted solves at a time.  An admitted solve holds device and pinned
-// memory; under overload, admitting everything the budget allows (tens of
-// thousands of small solves) only multiplies live allocations beyond what the
-// next launch windows can use.  The rest wait in the queue.
-constexpr size_t kMaxActive = 1024;
+constexpr int kConcurrentFinishes = 64;
+
+}  // namespace
+
+struct mcrt_server;
+
+namespace {
+
+// The device side of the scheduler: windows on one compute stream.
+class HipBackend final : public sched::Backend {
+ public:
+  explicit HipBackend(mcrt_server* S) : S_(S) {}
+  hipError_t init();
+  void shutdown();
+  int create(uint64_t job, int64_t* nchunks, double* cost, std::string* err) override;
+  void destroy(uint64_t job) override;
+  int launch(bool mixed, const std::vector<sched::Task>& tasks, std::string* err) override;
+  int poll(std::vector<int64_t>* states, sched::Time* start, std::string* err) override;
+  void preempt() override { __atomic_store_n(flag_, 1u, __ATOMIC_RELEASE); }
+  int64_t finish(const std::vector<uint64_t>& jobs, std::string* err) override;
+  int finished(int64_t handle, std::string* err) override;
+
+ private:
+  Ticket* ticket(uint64_t id);
+  mcrt_server* S_;
+  hipStream_t lane_ = nullptr, setup_ = nullptr;
+  hipEvent_t setup_ev_ = nullptr;   // after the last admitted solve's clear
+  hipEvent_t ref_ev_ = nullptr;     // device time zero, taken at ref_t_
+  Clock::time_point ref_t_;
+  hipEvent_t start_ev_ = nullptr, done_ev_ = nullptr;  // of the window in flight
+  // window staging: [next counter (64 B)] [chunks] [states], pinned and device
+  unsigned char* h_win_ = nullptr;
+  unsigned char* d_win_ = nullptr;
+  size_t win_bytes_ = 0;
+  int win_n_ = 0;
+  unsigned int* flag_ = nullptr;   // preempt flag (coherent pinned host memory)
+  unsigned int* d_flag_ = nullptr;
+  unsigned char* conts_ = nullptr;  // continuation blocks
+  size_t cont_bytes_ = 0;
+  struct FinSlot {
+    hipEvent_t ev = nullptr;
+    bool used = false;
+  } fins_[kConcurrentFinishes];
+};
 
 }  // namespace
 
@@ -208,37 +246,249 @@ struct mcrt_server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   bool stop = false;
-  std::deque<Ticket*> pending;
-  std::vector<Ticket*> active;
+  std::vector<Ticket*> incoming;
+  std::vector<uint64_t> cancels;
   std::map<uint64_t, Ticket*> tickets;
   uint64_t next_id = 1;
-  size_t budget = 0, in_use = 0;
+  std::atomic<double> slice_ms{8.0};
+  size_t budget = 0;
+  int blocks = 0;
   mcrt::TableCache* cache = nullptr;
   Pool* dev_pool = nullptr;
   Pool* host_pool = nullptr;
-  hipStream_t setup = nullptr;
-  hipEvent_t setup_ev = nullptr;  // after the last admitted solve's clear
-  struct Lane {
-    hipStream_t s = nullptr;
-    void* slot_mem = nullptr;
-    unsigned int* dnext = nullptr;
-    mcrt::BatchChunk* dchunks = nullptr;
-    mcrt::BatchChunk* hchunks = nullptr;  // pinned
-    hipEvent_t ev = nullptr;
-    bool busy = false;
-    std::vector<Ticket*> members, finishing;
-  } lanes[kLanes];
-  int blocks = 0;
-  bool prefer_mixed = true;
-  // statistics
+  HipBackend* backend = nullptr;
+  sched::Options opt;
+  // statistics (the scheduler's, copied after every step)
   mcrt_server_stats st{};
-  double busy_ms = 0.0;
 
   void run();
-  void admit(std::vector<Ticket*>& batch);
-  bool launch(Lane& L);
   void release(Ticket* t);
 };
+
+namespace {
+
+Ticket* HipBackend::ticket(uint64_t id) {
+  std::lock_guard<std::mutex> g(S_->mu);
+  auto it = S_->tickets.find(id);
+  return it == S_->tickets.end() ? nullptr : it->second;
+}
+
+hipError_t HipBackend::init() {
+  const int cap = S_->opt.window_tasks;
+  win_bytes_ = 64 + (sizeof(mcrt::BatchChunk) + sizeof(int64_t)) * (size_t)cap;
+  cont_bytes_ = std::max(mcrt::batch_slot_bytes(true, 1), mcrt::batch_slot_bytes(false, 1));
+  cont_bytes_ = (cont_bytes_ + 255) & ~size_t(255);
+  hipError_t e = hipStreamCreateWithFlags(&lane_, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&setup_, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&setup_ev_, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreate(&ref_ev_);
+  if (e == hipSuccess) e = hipEventCreate(&start_ev_);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&done_ev_, hipEventDisableTiming);
+  for (auto& f : fins_)
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&f.ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&h_win_, win_bytes_, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_win_, win_bytes_);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&flag_, 256, hipHostMallocCoherent);
+  if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&d_flag_, flag_, 0);
+  if (e == hipSuccess) e = hipMalloc((void**)&conts_, cont_bytes_ * (size_t)S_->opt.conts);
+  if (e == hipSuccess) {
+    std::memset(flag_, 0, 256);
+    e = hipEventRecord(ref_ev_, lane_);
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(ref_ev_);
+  ref_t_ = Clock::now();
+  return e;
+}
+
+void HipBackend::shutdown() {
+  if (lane_) (void)hipStreamSynchronize(lane_);
+  if (setup_) (void)hipStreamSynchronize(setup_);
+  for (auto& f : fins_)
+    if (f.ev) (void)hipEventDestroy(f.ev);
+  if (setup_ev_) (void)hipEventDestroy(setup_ev_);
+  if (ref_ev_) (void)hipEventDestroy(ref_ev_);
+  if (start_ev_) (void)hipEventDestroy(start_ev_);
+  if (done_ev_) (void)hipEventDestroy(done_ev_);
+  if (h_win_) (void)hipHostFree(h_win_);
+  if (d_win_) (void)hipFree(d_win_);
+  if (flag_) (void)hipHostFree(flag_);
+  if (conts_) (void)hipFree(conts_);
+  if (lane_) (void)hipStreamDestroy(lane_);
+  if (setup_) (void)hipStreamDestroy(setup_);
+}
+
+// Device state of a newly admitted solve (the clear runs on the set-up
+// stream; windows wait for setup_ev_).
+int HipBackend::create(uint64_t job, int64_t* nchunks, double* cost, std::string* err) {
+  Ticket* t = ticket(job);
+  if (!t) {
+    *err = "unknown ticket";
+    return MCRT_ERR_INVALID;
+  }
+  {
+    std::lock_guard<std::mutex> g(S_->mu);
+    t->admitted = true;
+    t->state = MCRT_TICKET_RUNNING;
+  }
+  t->released = false;
+  Pool* pool = S_->dev_pool;
+  auto alloc = [t, pool](size_t n, hipError_t* e) -> void* {
+    void* p = pool->get(n ? n : 8, e);
+    if (!p) return nullptr;
+    t->allocs.push_back({p, n ? n : 8});
+    return p;
+  };
+  const mcrt_request& q = t->r.req;
+  int rc = mcrt::solve_create(t->sd, S_->ctx->lds_per_block, q.cfg, q.geom, q.I0, q.seed, q.ray_begin,
+                              q.instance, t->mixed, q.chunk_rays, mcrt::ChunkRule::Latency, alloc, err,
+                              S_->cache);
+  hipError_t e = hipSuccess;
+  if (!rc) e = mcrt::solve_prepare(t->sd, setup_);
+  if (!rc && e == hipSuccess) e = hipEventRecord(setup_ev_, setup_);
+  if (!rc && e == hipSuccess) {
+    const size_t n = (size_t)t->sd.nout;
+    t->h_bytes = sizeof(double) * 2 * n + sizeof(unsigned long long) * mcrt::kCntN;
+    t->h_block = S_->host_pool->get(t->h_bytes, &e);
+    if (t->h_block) {
+      t->h_out = (double*)t->h_block;
+      t->h_sig = t->h_out + n;
+      t->h_cnt = (unsigned long long*)(t->h_sig + n);
+    }
+  }
+  if (!rc && e != hipSuccess) {
+    rc = e == hipErrorOutOfMemory ? MCRT_ERR_NOMEM : MCRT_ERR_HIP;
+    *err = hipGetErrorString(e);
+  }
+  *nchunks = rc ? 0 : t->sd.nchunks;
+  *cost = rc ? 0.0 : t->sd.cost;
+  return rc;
+}
+
+void HipBackend::destroy(uint64_t job) {
+  Ticket* t = ticket(job);
+  if (t) S_->release(t);
+}
+
+int HipBackend::launch(bool mixed, const std::vector<sched::Task>& tasks, std::string* err) {
+  const int n = (int)tasks.size();
+  if (n <= 0 || n > S_->opt.window_tasks) {
+    *err = "window size out of range";
+    return MCRT_ERR_INVALID;
+  }
+  // staging: next = 0, the chunk records, the outcome words (-1: unclaimed)
+  std::memset(h_win_, 0, 64);
+  auto* hc = (mcrt::BatchChunk*)(h_win_ + 64);
+  auto* hs = (int64_t*)(h_win_ + 64 + sizeof(mcrt::BatchChunk) * (size_t)n);
+  size_t lds = 0;
+  for (int i = 0; i < n; ++i) {
+    const sched::Task& k = tasks[i];
+    Ticket* t = ticket(k.job);
+    if (!t || k.cont < 0 || k.cont >= S_->opt.conts || k.chunk < 0 || k.chunk >= t->sd.nchunks ||
+        t->sd.mixed != mixed) {
+      *err = "malformed window task";
+      return MCRT_ERR_INVALID;
+    }
+    mcrt::solve_chunk(t->sd, k.chunk, &hc[i]);
+    if (k.resume) {
+      if (k.rb < hc[i].rb || k.rb > hc[i].re) {
+        *err = "resume point outside its chunk";
+        return MCRT_ERR_INVALID;
+      }
+      hc[i].rb = k.rb;
+      hc[i].resume = 1;
+    }
+    hc[i].cont = conts_ + cont_bytes_ * (size_t)k.cont;
+    hs[i] = sched::kUnclaimed;
+    lds = std::max(lds, t->sd.lds_bytes);
+  }
+  const size_t bytes = 64 + (sizeof(mcrt::BatchChunk) + sizeof(int64_t)) * (size_t)n;
+  auto* dnext = (unsigned int*)d_win_;
+  auto* dc = (const mcrt::BatchChunk*)(d_win_ + 64);
+  auto* ds = (int64_t*)(d_win_ + 64 + sizeof(mcrt::BatchChunk) * (size_t)n);
+  // the flag is clear before the launch is enqueued (no window is in flight)
+  __atomic_store_n(flag_, 0u, __ATOMIC_RELEASE);
+  hipError_t e = hipStreamWaitEvent(lane_, setup_ev_, 0);  // the solves' slabs are clear
+  if (e == hipSuccess) e = hipMemcpyAsync(d_win_, h_win_, bytes, hipMemcpyHostToDevice, lane_);
+  if (e == hipSuccess) e = hipEventRecord(start_ev_, lane_);
+  if (e == hipSuccess)
+    e = mcrt::batch_launch(mixed, dc, n, dnext, nullptr, S_->blocks, lds, lane_, ds, d_flag_, false);
+  if (e == hipSuccess) e = hipMemcpyAsync(hs, ds, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost, lane_);
+  if (e == hipSuccess) e = hipEventRecord(done_ev_, lane_);
+  if (e != hipSuccess) {
+    // an error after the first enqueue: wait for what was enqueued, then fail
+    (void)hipStreamSynchronize(lane_);
+    *err = hipGetErrorString(e);
+    return MCRT_ERR_HIP;
+  }
+  win_n_ = n;
+  return MCRT_OK;
+}
+
+int HipBackend::poll(std::vector<int64_t>* states, sched::Time* start, std::string* err) {
+  const hipError_t q = hipEventQuery(done_ev_);
+  if (q == hipErrorNotReady) return 0;
+  if (q != hipSuccess) {
+    *err = hipGetErrorString(q);
+    return MCRT_ERR_HIP;
+  }
+  const auto* hs = (const int64_t*)(h_win_ + 64 + sizeof(mcrt::BatchChunk) * (size_t)win_n_);
+  states->assign(hs, hs + win_n_);
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ref_ev_, start_ev_) == hipSuccess)
+    *start = ref_t_ + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double, std::milli>(ms));
+  return 1;
+}
+
+int64_t HipBackend::finish(const std::vector<uint64_t>& jobs, std::string* err) {
+  int slot = -1;
+  for (int i = 0; i < kConcurrentFinishes; ++i)
+    if (!fins_[i].used) {
+      slot = i;
+      break;
+    }
+  if (slot < 0) {
+    *err = "too many read-backs in flight";
+    return MCRT_ERR_HIP;
+  }
+  hipError_t e = hipSuccess;
+  for (uint64_t id : jobs) {
+    Ticket* t = ticket(id);
+    if (!t) continue;
+    const mcrt::SolveDev& sd = t->sd;
+    const size_t n = (size_t)sd.nout;
+    if (e == hipSuccess) e = mcrt::solve_finish(sd, lane_);
+    if (e == hipSuccess) e = hipMemcpyAsync(t->h_out, sd.out, sizeof(double) * n, hipMemcpyDeviceToHost, lane_);
+    if (e == hipSuccess) e = hipMemcpyAsync(t->h_sig, sd.sig, sizeof(double) * n, hipMemcpyDeviceToHost, lane_);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(t->h_cnt, sd.counters, sizeof(unsigned long long) * mcrt::kCntN,
+                         hipMemcpyDeviceToHost, lane_);
+  }
+  if (e == hipSuccess) e = hipEventRecord(fins_[slot].ev, lane_);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(lane_);
+    *err = hipGetErrorString(e);
+    return MCRT_ERR_HIP;
+  }
+  fins_[slot].used = true;
+  return slot;
+}
+
+int HipBackend::finished(int64_t h, std::string* err) {
+  if (h < 0 || h >= kConcurrentFinishes || !fins_[h].used) {
+    *err = "unknown read-back";
+    return MCRT_ERR_INVALID;
+  }
+  const hipError_t q = hipEventQuery(fins_[h].ev);
+  if (q == hipErrorNotReady) return 0;
+  fins_[h].used = false;
+  if (q != hipSuccess) {
+    *err = hipGetErrorString(q);
+    return MCRT_ERR_HIP;
+  }
+  return 1;
+}
+
+}  // namespace
 
 void mcrt_server::release(Ticket* t) {
   if (t->released) return;
@@ -247,268 +497,75 @@ void mcrt_server::release(Ticket* t) {
   for (auto& a : t->allocs) dev_pool->put(a.first, a.second);
   t->allocs.clear();
   t->released = true;
-  if (t->admitted) {
-    std::lock_guard<std::mutex> g(mu);
-    in_use -= t->need;
-    t->admitted = false;
-  }
-}
-
-// Create the device state of newly admitted solves (outside the lock).
-void mcrt_server::admit(std::vector<Ticket*>& batch) {
-  for (Ticket* t : batch) {
-    Pool* pool = dev_pool;
-    auto alloc = [t, pool](size_t n, hipError_t* e) -> void* {
-      void* p = pool->get(n ? n : 8, e);
-      if (!p) return nullptr;
-      t->allocs.push_back({p, n ? n : 8});
-      return p;
-    };
-    std::string err;
-    const mcrt_request& q = t->r.req;
-    int rc = mcrt::solve_create(t->sd, ctx->lds_per_block, q.cfg, q.geom, q.I0, q.seed, q.ray_begin,
-                                q.instance, t->mixed, q.chunk_rays, mcrt::ChunkRule::Latency,
-                                alloc, &err, cache);
-    hipError_t e = hipSuccess;
-    // the clear runs on the set-up stream; launches wait for setup_ev
-    if (!rc) e = mcrt::solve_prepare(t->sd, setup);
-    if (!rc && e == hipSuccess) {
-      const size_t n = (size_t)t->sd.nout;
-      t->h_bytes = sizeof(double) * 2 * n + sizeof(unsigned long long) * mcrt::kCntN;
-      t->h_block = host_pool->get(t->h_bytes, &e);
-      if (t->h_block) {
-        t->h_out = (double*)t->h_block;
-        t->h_sig = t->h_out + n;
-        t->h_cnt = (unsigned long long*)(t->h_sig + n);
-      }
-    }
-    if (!rc && e != hipSuccess) {
-      rc = e == hipErrorOutOfMemory ? MCRT_ERR_NOMEM : MCRT_ERR_HIP;
-      err = hipGetErrorString(e);
-    }
-    if (rc) {
-      release(t);
-      std::lock_guard<std::mutex> g(mu);
-      t->state = MCRT_TICKET_FAILED;
-      t->err_code = rc;
-      t->err = err;
-      t->t_done = Clock::now();
-      ++st.failed;
-      cv_done.notify_all();
-      continue;
-    }
-    std::lock_guard<std::mutex> g(mu);
-    t->state = MCRT_TICKET_RUNNING;
-    t->t_start = Clock::now();
-    active.push_back(t);
-  }
-  (void)hipEventRecord(setup_ev, setup);
-}
-
-// Fill and launch one window on lane L; false when there is nothing to run.
-bool mcrt_server::launch(Lane& L) {
-  std::vector<Ticket*> elig;
-  bool mixed;
-  {
-    std::lock_guard<std::mutex> g(mu);
-    bool any[2] = {false, false};
-    for (Ticket* t : active)
-      if (!t->cancel && t->next_chunk < t->sd.nchunks) any[t->mixed ? 1 : 0] = true;
-    if (!any[0] && !any[1]) return false;
-    mixed = any[1] && (prefer_mixed || !any[0]);
-    prefer_mixed = !mixed;  // alternate when both precisions wait
-    for (Ticket* t : active)
-      if (!t->cancel && t->mixed == mixed && t->next_chunk < t->sd.nchunks) elig.push_back(t);
-  }
-  // round-robin over the solves (oldest first) up to ~2 chunks per workgroup
-  const int target = std::min(kLaneChunks, std::max(2 * blocks, (int)elig.size()));
-  struct Item { double cost; Ticket* t; int64_t k; };
-  std::vector<Item> items;
-  std::vector<int64_t> take(elig.size(), 0);
-  for (bool more = true; more && (int)items.size() < target;) {
-    more = false;
-    for (size_t i = 0; i < elig.size() && (int)items.size() < target; ++i) {
-      Ticket* t = elig[i];
-      const int64_t k = t->next_chunk + take[i];
-      if (k >= t->sd.nchunks) continue;
-      items.push_back({t->sd.cost, t, k});
-      ++take[i];
-      more = true;
-    }
-  }
-  if (items.empty()) return false;
-  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
-  size_t lds = 0;
-  L.members.clear();
-  L.finishing.clear();
-  {
-    std::lock_guard<std::mutex> g(mu);  // mcrt_cancel reads next_chunk
-    for (size_t i = 0; i < elig.size(); ++i) {
-      if (!take[i]) continue;
-      Ticket* t = elig[i];
-      t->next_chunk += take[i];
-      ++t->inflight;
-      L.members.push_back(t);
-      if (t->next_chunk >= t->sd.nchunks) L.finishing.push_back(t);
-      lds = std::max(lds, t->sd.lds_bytes);
-    }
-  }
-  for (size_t j = 0; j < items.size(); ++j) mcrt::solve_chunk(items[j].t->sd, items[j].k, &L.hchunks[j]);
-  hipError_t e = hipMemcpyAsync(L.dchunks, L.hchunks, sizeof(mcrt::BatchChunk) * items.size(),
-                                hipMemcpyHostToDevice, L.s);
-  if (e == hipSuccess) e = hipStreamWaitEvent(L.s, setup_ev, 0);  // the solves' slabs are clear
-  if (e == hipSuccess)
-    e = mcrt::batch_launch(mixed, L.dchunks, (int)items.size(), L.dnext, L.slot_mem, blocks, lds, L.s);
-  // a solve finishing here may still have chunks in the other lane's launch:
-  // its reduction waits for that launch
-  for (Lane& O : lanes) {
-    if (&O == &L || !O.busy || e != hipSuccess) continue;
-    bool shared = false;
-    for (Ticket* t : L.finishing)
-      if (std::find(O.members.begin(), O.members.end(), t) != O.members.end()) shared = true;
-    if (shared) e = hipStreamWaitEvent(L.s, O.ev, 0);
-  }
-  for (Ticket* t : L.finishing) {
-    if (e != hipSuccess) break;
-    e = mcrt::solve_finish(t->sd, L.s);
-    const size_t n = (size_t)t->sd.nout;
-    if (e == hipSuccess) e = hipMemcpyAsync(t->h_out, t->sd.out, sizeof(double) * n, hipMemcpyDeviceToHost, L.s);
-    if (e == hipSuccess) e = hipMemcpyAsync(t->h_sig, t->sd.sig, sizeof(double) * n, hipMemcpyDeviceToHost, L.s);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(t->h_cnt, t->sd.counters, sizeof(unsigned long long) * mcrt::kCntN,
-                         hipMemcpyDeviceToHost, L.s);
-  }
-  if (e == hipSuccess) e = hipEventRecord(L.ev, L.s);
-  if (e != hipSuccess) {
-    // the lane is unusable: its solves launch nothing more, and the sweep
-    // publishes them as failed once no launch holds them (a ticket is never
-    // published while the server still lists it: its owner may free it)
-    (void)hipStreamSynchronize(L.s);
-    std::lock_guard<std::mutex> g(mu);
-    for (Ticket* t : L.members) {
-      --t->inflight;
-      t->cancel = true;
-      if (!t->err_code) {
-        t->err_code = MCRT_ERR_HIP;
-        t->err = hipGetErrorString(e);
-      }
-    }
-    L.members.clear();
-    L.finishing.clear();
-    return false;
-  }
-  L.busy = true;
-  ++st.launches;
-  st.chunks += items.size();
-  return true;
 }
 
 void mcrt_server::run() {
   (void)hipSetDevice(ctx->device);
-  auto t_busy = Clock::now();
-  bool was_busy = false;
-  for (;;) {
-    bool progress = false;
-    // ---- admission
-    std::vector<Ticket*> batch;
-    {
-      std::lock_guard<std::mutex> g(mu);
-      while (!pending.empty()) {
-        Ticket* t = pending.front();
-        if (t->cancel) {
-          pending.pop_front();
-          t->state = MCRT_TICKET_CANCELLED;
-          t->t_done = Clock::now();
-          ++st.cancelled;
-          cv_done.notify_all();
-          continue;
-        }
-        if (t->need > budget) {
-          pending.pop_front();
-          t->state = MCRT_TICKET_FAILED;
-          t->err_code = MCRT_ERR_NOMEM;
-          t->err = "the solve needs more device memory than the server's budget";
-          t->t_done = Clock::now();
-          ++st.failed;
-          cv_done.notify_all();
-          continue;
-        }
-        if (in_use + t->need > budget) break;  // wait for memory to free up
-        if (active.size() + batch.size() >= kMaxActive) break;
-        pending.pop_front();
-        in_use += t->need;
-        t->admitted = true;
-        batch.push_back(t);
-      }
-    }
-    if (!batch.empty()) {
-      admit(batch);
-      progress = true;
-    }
-    // ---- completions
-    for (Lane& L : lanes) {
-      if (!L.busy || hipEventQuery(L.ev) != hipSuccess) continue;
-      L.busy = false;
-      progress = true;
-      std::lock_guard<std::mutex> g(mu);
-      for (Ticket* t : L.members) --t->inflight;
-      for (Ticket* t : L.finishing) t->finished = true;
-      L.members.clear();
-      L.finishing.clear();
-    }
-    // ---- solves that will launch nothing more: free their device memory,
-    // then publish the final state
-    {
-      std::vector<Ticket*> over;
-      {
-        std::lock_guard<std::mutex> g(mu);
-        for (auto it = active.begin(); it != active.end();) {
-          Ticket* t = *it;
-          if (t->inflight == 0 && (t->finished || t->cancel)) {
-            over.push_back(t);
-            it = active.erase(it);
-          } else {
-            ++it;
-          }
-        }
-      }
-      for (Ticket* t : over) release(t);
-      if (!over.empty()) {
-        progress = true;
-        std::lock_guard<std::mutex> g(mu);
-        for (Ticket* t : over) {
-          t->t_done = Clock::now();
-          if (t->err_code) {  // a launch holding its chunks failed
-            t->state = MCRT_TICKET_FAILED;
-            ++st.failed;
-          } else if (t->finished) {
-            t->state = MCRT_TICKET_DONE;
-            ++st.completed;
-            st.rays += (uint64_t)t->sd.N;
-          } else {
-            t->state = MCRT_TICKET_CANCELLED;
-            ++st.cancelled;
-          }
-        }
-        cv_done.notify_all();
-      }
-    }
-    // ---- launches
-    for (Lane& L : lanes)
-      if (!L.busy && launch(L)) progress = true;
-    const bool busy = lanes[0].busy || lanes[1].busy;
-    if (busy && !was_busy) t_busy = Clock::now();
-    if (!busy && was_busy) busy_ms += std::chrono::duration<double, std::milli>(Clock::now() - t_busy).count();
-    was_busy = busy;
-    if (progress) continue;
-    std::unique_lock<std::mutex> lk(mu);
-    if (stop && pending.empty() && active.empty() && !busy) break;
-    if (busy) {
-      lk.unlock();
-      std::this_thread::sleep_for(std::chrono::microseconds(200));
+  // a ticket is published once, after the scheduler has freed its device
+  // state and nothing in flight holds it (its owner may then free it)
+  auto publish = [this](const sched::Job& j, sched::Final f) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = tickets.find(j.id);
+    if (it == tickets.end()) return;
+    Ticket* t = it->second;
+    t->t_admit = j.admitted ? j.t_admit : j.t_done;
+    t->started = j.started;
+    t->t_start = j.started ? j.t_start : j.t_done;
+    t->t_done = j.t_done;
+    if (f == sched::Final::Done) {
+      t->state = MCRT_TICKET_DONE;
+      st.rays += (uint64_t)t->sd.N;
+    } else if (f == sched::Final::Cancelled) {
+      t->state = MCRT_TICKET_CANCELLED;
     } else {
+      t->state = MCRT_TICKET_FAILED;
+      t->err_code = j.err_code ? j.err_code : MCRT_ERR_HIP;
+      t->err = j.err;
+    }
+    cv_done.notify_all();
+  };
+  sched::Scheduler sc(backend, opt, publish);
+  bool stopping = false;
+  for (;;) {
+    std::vector<Ticket*> in;
+    std::vector<uint64_t> cn;
+    {
+      std::lock_guard<std::mutex> g(mu);
+      in.swap(incoming);
+      cn.swap(cancels);
+      if (stop && !stopping) stopping = true;
+    }
+    const auto now = Clock::now();
+    for (Ticket* t : in) sc.submit(t->id, t->mixed, t->need, t->t_submit);
+    for (uint64_t id : cn) sc.cancel(id);
+    if (stopping) sc.cancel_all();
+    sc.set_slice(slice_ms.load());
+    const bool progress = sc.step(now);
+    {
+      const sched::Stats s = sc.stats();
+      std::lock_guard<std::mutex> g(mu);
+      st.submitted = s.submitted;
+      st.completed = s.completed;
+      st.cancelled = s.cancelled;
+      st.failed = s.failed;
+      st.launches = s.launches;
+      st.chunks = s.tasks;
+      st.queued = s.queued + incoming.size();
+      st.running = s.running;
+      st.device_bytes_in_use = (int64_t)s.in_use;
+      st.busy_ms = s.busy_ms;
+      st.preemptions = s.preemptions;
+      st.resumed = s.resumed;
+    }
+    if (stopping && sc.idle()) break;
+    if (progress) continue;
+    if (sc.in_flight()) {
+      // a stopping window ends within an iteration: look again sooner
+      std::this_thread::sleep_for(std::chrono::microseconds(sc.preempt_pending() ? 20 : 100));
+    } else {
+      std::unique_lock<std::mutex> lk(mu);
       cv_work.wait_for(lk, std::chrono::milliseconds(20),
-                       [this] { return stop || !pending.empty(); });
+                       [this] { return stop || !incoming.empty() || !cancels.empty(); });
     }
   }
 }
@@ -529,18 +586,15 @@ extern "C" int mcrt_server_create(mcrt_ctx* ctx, int64_t device_budget_bytes,
   S->blocks = std::max(1, ctx->cu_count);
   S->dev_pool = new Pool(false, (size_t)4 << 30);
   S->host_pool = new Pool(true, (size_t)1 << 30);
-  const size_t slot = std::max(mcrt::batch_slot_bytes(true, S->blocks), mcrt::batch_slot_bytes(false, S->blocks));
-  e = hipStreamCreateWithFlags(&S->setup, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&S->setup_ev, hipEventDisableTiming);
-  for (auto& L : S->lanes) {
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&L.slot_mem, slot);
-    if (e == hipSuccess) e = hipMalloc((void**)&L.dnext, sizeof(unsigned int));
-    if (e == hipSuccess) e = hipMalloc((void**)&L.dchunks, sizeof(mcrt::BatchChunk) * kLaneChunks);
-    if (e == hipSuccess)
-      e = hipHostMalloc((void**)&L.hchunks, sizeof(mcrt::BatchChunk) * kLaneChunks, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.ev, hipEventDisableTiming);
-  }
+  S->opt.blocks = S->blocks;
+  S->opt.window_tasks = 2 * S->blocks;
+  S->opt.conts = 3 * S->blocks;
+  S->opt.max_active = 1024;
+  S->opt.budget = S->budget;
+  S->opt.slice_ms = S->slice_ms.load();
+  S->st.device_budget = (int64_t)S->budget;
+  S->backend = new HipBackend(S);
+  e = S->backend->init();
   // every solve's LDS fits in the largest request, so set it once
   if (e == hipSuccess) e = mcrt::batch_set_lds(true, ctx->lds_per_block - 16);
   if (e == hipSuccess) e = mcrt::batch_set_lds(false, ctx->lds_per_block - 16);
@@ -580,8 +634,7 @@ extern "C" int mcrt_submit(mcrt_server* S, const mcrt_request* q, uint64_t* tick
     }
     t->id = S->next_id++;
     S->tickets[t->id] = t;
-    S->pending.push_back(t);
-    ++S->st.submitted;
+    S->incoming.push_back(t);
   }
   S->cv_work.notify_one();
   *ticket = t->id;
@@ -623,6 +676,9 @@ extern "C" int mcrt_result(mcrt_server* S, uint64_t ticket, double A[5], double*
     if (t->state < MCRT_TICKET_DONE) return fail(nullptr, MCRT_ERR_NOT_READY, "ticket %llu is not finished", (unsigned long long)ticket);
     S->tickets.erase(it);
   }
+  auto ms = [](Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
   int rc = MCRT_OK;
   if (t->state == MCRT_TICKET_DONE) {
     const mcrt::SolveDev& sd = t->sd;
@@ -642,10 +698,12 @@ extern "C" int mcrt_result(mcrt_server* S, uint64_t ticket, double A[5], double*
       stats->cb_evals_single = c[mcrt::kCntCb1];
       stats->rejections = c[mcrt::kCntRej];
       stats->nonfinite = c[mcrt::kCntNonfinite];
-      stats->kernel_ms = std::chrono::duration<double, std::milli>(t->t_done - t->t_start).count();
-      stats->total_ms = std::chrono::duration<double, std::milli>(t->t_done - t->t_submit).count();
+      stats->kernel_ms = ms(t->t_start, t->t_done);
+      stats->total_ms = ms(t->t_submit, t->t_done);
       stats->batches = (uint64_t)sd.nbatch;
       stats->chunks = (uint64_t)sd.nchunks;
+      stats->queue_ms = ms(t->t_submit, t->t_admit);
+      stats->wait_ms = ms(t->t_admit, t->t_start);
     }
   } else if (t->state == MCRT_TICKET_CANCELLED) {
     rc = fail(nullptr, MCRT_ERR_CANCELLED, "ticket %llu was cancelled", (unsigned long long)ticket);
@@ -653,8 +711,8 @@ extern "C" int mcrt_result(mcrt_server* S, uint64_t ticket, double A[5], double*
     rc = fail(nullptr, t->err_code ? t->err_code : MCRT_ERR_HIP, "ticket %llu failed: %s",
               (unsigned long long)ticket, t->err.c_str());
   }
-  if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(t->t_done - t->t_submit).count();
-  S->host_pool->put(t->h_block, t->h_bytes);
+  if (latency_ms) *latency_ms = ms(t->t_submit, t->t_done);
+  if (t->h_block) S->host_pool->put(t->h_block, t->h_bytes);
   delete t;
   return rc;
 }
@@ -665,8 +723,7 @@ extern "C" int mcrt_cancel(mcrt_server* S, uint64_t ticket) {
     std::lock_guard<std::mutex> g(S->mu);
     auto it = S->tickets.find(ticket);
     if (it == S->tickets.end()) return fail(nullptr, MCRT_ERR_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
-    Ticket* t = it->second;
-    if (t->state < MCRT_TICKET_DONE && t->next_chunk < (t->sd.nchunks ? t->sd.nchunks : 1)) t->cancel = true;
+    if (it->second->state < MCRT_TICKET_DONE) S->cancels.push_back(ticket);
   }
   S->cv_work.notify_one();
   return MCRT_OK;
@@ -676,14 +733,17 @@ extern "C" int mcrt_server_stats_get(mcrt_server* S, mcrt_server_stats* out) {
   if (!S || !out) return fail(nullptr, MCRT_ERR_INVALID, "server_stats: NULL argument");
   std::lock_guard<std::mutex> g(S->mu);
   *out = S->st;
-  out->queued = S->pending.size();
-  out->running = S->active.size();
-  out->device_bytes_in_use = (int64_t)S->in_use;
   out->device_budget = (int64_t)S->budget;
-  out->busy_ms = S->busy_ms;
   size_t cb = 0;
   mcrt::table_cache_stats(S->cache, &cb, &out->table_hits, &out->table_misses);
   out->table_cache_bytes = (int64_t)cb;
+  return MCRT_OK;
+}
+
+extern "C" int mcrt_server_set_slice(mcrt_server* S, double slice_ms) {
+  if (!S) return fail(nullptr, MCRT_ERR_INVALID, "server is NULL");
+  if (!(slice_ms >= 0.0 && slice_ms <= 1e6)) return fail(S->ctx, MCRT_ERR_INVALID, "slice_ms out of range");
+  S->slice_ms.store(slice_ms);
   return MCRT_OK;
 }
 
@@ -692,30 +752,18 @@ extern "C" void mcrt_server_destroy(mcrt_server* S) {
   {
     std::lock_guard<std::mutex> g(S->mu);
     S->stop = true;
-    for (Ticket* t : S->pending) t->cancel = true;
-    for (Ticket* t : S->active) t->cancel = true;
   }
   S->cv_work.notify_all();
   if (S->worker.joinable()) S->worker.join();
   (void)hipSetDevice(S->ctx->device);
+  if (S->backend) S->backend->shutdown();
   for (auto& kv : S->tickets) {
     Ticket* t = kv.second;
     S->release(t);
-    S->host_pool->put(t->h_block, t->h_bytes);
+    if (t->h_block) S->host_pool->put(t->h_block, t->h_bytes);
     delete t;
   }
-  for (auto& L : S->lanes) {
-    if (L.s) (void)hipStreamSynchronize(L.s);
-    if (L.ev) (void)hipEventDestroy(L.ev);
-    if (L.hchunks) (void)hipHostFree(L.hchunks);
-    if (L.dchunks) (void)hipFree(L.dchunks);
-    if (L.dnext) (void)hipFree(L.dnext);
-    if (L.slot_mem) (void)hipFree(L.slot_mem);
-    if (L.s) (void)hipStreamDestroy(L.s);
-  }
-  if (S->setup) (void)hipStreamSynchronize(S->setup);
-  if (S->setup_ev) (void)hipEventDestroy(S->setup_ev);
-  if (S->setup) (void)hipStreamDestroy(S->setup);
+  delete S->backend;
   mcrt::table_cache_destroy(S->cache);
   delete S->dev_pool;
   delete S->host_pool;

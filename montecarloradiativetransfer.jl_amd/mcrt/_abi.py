@@ -121,6 +121,8 @@ class mcrt_stats(C.Structure):
         ("total_ms", C.c_double),
         ("batches", C.c_uint64),
         ("chunks", C.c_uint64),
+        ("queue_ms", C.c_double),
+        ("wait_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -153,6 +155,8 @@ class mcrt_server_stats(C.Structure):
         ("table_hits", C.c_uint64),
         ("table_misses", C.c_uint64),
         ("busy_ms", C.c_double),
+        ("preemptions", C.c_uint64),
+        ("resumed", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -218,6 +222,7 @@ SIGNATURES = {
     "mcrt_cancel": (C.c_int, [_vp, C.c_uint64]),
     "mcrt_server_stats_get": (C.c_int, [_vp, C.POINTER(mcrt_server_stats)]),
     "mcrt_server_destroy": (None, [_vp]),
+    "mcrt_server_set_slice": (C.c_int, [_vp, C.c_double]),
     "mcrt_plan_destroy": (None, [_vp]),
 }
 

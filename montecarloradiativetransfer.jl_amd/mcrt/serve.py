@@ -92,6 +92,11 @@ class Server:
     def cancel(self, ticket: int) -> None:
         _abi.check(self.ctx.lib.mcrt_cancel(self.handle, C.c_uint64(ticket)))
 
+    def set_slice(self, ms: float) -> None:
+        """How long a launch window keeps the device while other work waits
+        (0: stop it as soon as work waits)."""
+        _abi.check(self.ctx.lib.mcrt_server_set_slice(self.handle, float(ms)))
+
     def stats(self) -> dict:
         s = _abi.mcrt_server_stats()
         _abi.check(self.ctx.lib.mcrt_server_stats_get(self.handle, C.byref(s)))
@@ -134,6 +139,14 @@ def latency_summary(lat_ms) -> dict:
             "max_ms": float(a.max())}
 
 
+def _pcts(v) -> dict:
+    a = np.asarray(v, float)
+    if a.size == 0:
+        return {}
+    return {"p50": float(np.percentile(a, 50)), "p99": float(np.percentile(a, 99)),
+            "mean": float(a.mean())}
+
+
 def poisson_load(srv: Server, pool: list[Request], *, rate: float, duration: float,
                  seed: int = 5, keep: int = 0, log=None) -> dict:
     """Submit requests drawn uniformly from `pool` at Poisson arrival times;
@@ -145,6 +158,7 @@ def poisson_load(srv: Server, pool: list[Request], *, rate: float, duration: flo
     tickets: list[tuple[int, Request]] = []
     done = threading.Event()
     lat, rays, kept = [], [], []
+    parts = {"queue_ms": [], "wait_ms": [], "device_ms": []}
     errors = []
 
     def collect():
@@ -165,6 +179,9 @@ def poisson_load(srv: Server, pool: list[Request], *, rate: float, duration: flo
                 continue
             lat.append(st["latency_ms"])
             rays.append(st["rays"])
+            parts["queue_ms"].append(st["queue_ms"])
+            parts["wait_ms"].append(st["wait_ms"])
+            parts["device_ms"].append(st["kernel_ms"])
             if len(kept) < keep:
                 kept.append((r, out))
             i += 1
@@ -190,6 +207,9 @@ def poisson_load(srv: Server, pool: list[Request], *, rate: float, duration: flo
            "submit_s": t_submit, "wall_s": wall, "completed": len(lat), "errors": len(errors),
            "rays": int(sum(rays)), "rays_per_s": float(sum(rays) / wall)}
     out.update(latency_summary(lat))
+    # per request: submit → admission, admission → device start of the first
+    # window that ran one of its chunks, that start → completion
+    out["breakdown"] = {k: _pcts(v) for k, v in parts.items()}
     out["server"] = srv.stats()
     out["kept"] = kept
     return out

@@ -148,3 +148,63 @@ def test_no_cpu_fallback():
     g = mcrt.SphereGeometry(R=2.0, scatterer=mcrt.RayleighScatterer(0.9, 0.5))
     with pytest.raises(NotImplementedError):
         mcrt.simulate(mcrt.Config(g), g, [1, -1, 0, 0], use_gpu=False)
+
+
+JULIA_TYPES = {"Int32": (4, 4), "UInt32": (4, 4), "Int64": (8, 8), "UInt64": (8, 8),
+               "Float64": (8, 8), "NTuple{4, Float64}": (32, 8)}
+JULIA_STRUCTS = {"McrtScatterer": "mcrt_scatterer", "McrtLayer": "mcrt_layer",
+                 "McrtGeometry": "mcrt_geometry", "McrtConfig": "mcrt_config",
+                 "McrtStats": "mcrt_stats", "McrtRequest": "mcrt_request"}
+
+
+def test_julia_binding_mirrors_c_layout(tmp_path):
+    """julia/src/hip.jl declares the C structs field for field: same names in
+    the same order, and C alignment of its field types gives the offsets and
+    sizes of include/mcrt.h (Julia is not installed here, so the binding is
+    checked by its text)."""
+    src = (ROOT / "julia" / "src" / "hip.jl").read_text()
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    parsed = {}
+    for jname, cname in JULIA_STRUCTS.items():
+        m = re.search(rf"^struct {jname}\b.*?\n(.*?)^end", src, flags=re.S | re.M)
+        assert m, jname
+        fields = []
+        for line in m.group(1).splitlines():
+            line = line.split("#")[0].strip()
+            if not line:
+                continue
+            name, typ = [x.strip() for x in line.split("::", 1)]
+            fields.append((name, typ))
+        off, align = 0, 1
+        layout = []
+        for name, typ in fields:
+            size, al = (8, 8) if typ.startswith("Ptr{") else JULIA_TYPES[typ]
+            off = (off + al - 1) // al * al
+            layout.append((name, off))
+            off += size
+            align = max(align, al)
+        parsed[cname] = (layout, (off + align - 1) // align * align)
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for name, _ in layout:
+            lines.append(f'printf("{cname}.{name} %zu\\n", offsetof({cname}, {name}));')
+    lines.append("return 0;}")
+    c = tmp_path / "jl_layout.c"
+    c.write_text("\n".join(lines))
+    exe = tmp_path / "jl_layout"
+    subprocess.run(["gcc", "-o", str(exe), str(c)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                  check=True).stdout.splitlines())
+    for cname, (layout, size) in parsed.items():
+        assert int(got[cname]) == size, cname
+        for name, off in layout:
+            assert int(got[f"{cname}.{name}"]) == off, (cname, name)
+        # every C field is mirrored
+        assert len(layout) == len(getattr(_abi, cname)._fields_), cname
+
+
+def test_julia_binding_calls_only_declared_functions():
+    src = (ROOT / "julia" / "src" / "hip.jl").read_text()
+    called = set(re.findall(r"ccall\(\(:(mcrt_[a-z0-9_]+),", src))
+    assert {"mcrt_simulate", "mcrt_context_create", "mcrt_last_error", "mcrt_table_upload",
+            "mcrt_table_release", "mcrt_batch_create", "mcrt_batch_read"} <= called
+    assert called <= set(declared_functions()), called - set(declared_functions())

@@ -135,13 +135,13 @@ def test_launch_shapes_and_autotuner():
 def _expected_chunks(N, tau, omega, bins, cb_bins, rule, min_int=1e-6, maxscat=10000):
     """The default chunking rules as include/mcrt.h states them: batches cut by
     ray count (ceil(N/64) within 512..8192), the server by work (about 4e7 bin
-    evaluations per chunk, 64..8192 rays); at least two chunks."""
+    evaluations per chunk, 64..8192 rays, with the events per ray of a weight
+    kept at ω·(1 - p) per event, p = 1/(1 + 0.75·τ^1.15), down to
+    minimum_intensity); at least two chunks."""
     import math
     if rule == "latency":
-        ev = 1.0 + 2.0 * tau + 0.5 * tau * tau
-        if 0.0 < omega < 1.0:
-            ev = min(ev, 1.0 + math.log(1.0 / min_int) / -math.log(omega))
-        ev = min(ev, maxscat)
+        keep = omega * (1.0 - 1.0 / (1.0 + 0.75 * tau ** 1.15))
+        ev = min(maxscat, 1.0 + math.log(1.0 / min_int) / -math.log(keep))
         c = int(max(64.0, min(8192.0, math.ceil(4e7 / (ev * (bins + 3 * cb_bins + 1))))))
     else:
         c = max(512, min(8192, -(-N // 64)))
@@ -175,7 +175,7 @@ def test_default_chunks_of_batches_and_served_solves():
     reqs, args = _chunk_cases()
     want_b = [_expected_chunks(*a, rule="throughput") for a in args]
     want_s = [_expected_chunks(*a, rule="latency") for a in args]
-    assert want_b == [8, 2, 2] and want_s == [4, 2, 16]
+    assert want_b == [8, 2, 2] and want_s == [16, 2, 16]
     with Batch(reqs) as b:
         assert b.info()["chunks"] == sum(want_b)
         b.launch()

@@ -96,3 +96,35 @@ def test_poisson_load_small():
     print(f"[serve/load] {rep}", flush=True)
     assert rep["completed"] == rep["requests"] and rep["errors"] == 0
     assert rep["p99_ms"] >= rep["p50_ms"] > 0
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_result_is_the_same_when_windows_are_cut(precision):
+    """A solve whose windows are stopped again and again by arriving work
+    resumes its chunks exactly where they stopped: the result equals the
+    batched launch of the same chunks up to the order of FP64 additions."""
+    cfg, g, _ = W.sphere_mie_cb(32768 if precision == "mixed" else 8192, Ntheta_table=1801)
+    big = Request(cfg, g, I0, seed=21, precision=precision, chunk_rays=512)
+    small = W.plane_rayleigh_tau1(1024, compute_cb=True)
+    import time
+    with Server() as srv:
+        srv.set_slice(0.0)  # stop the running window whenever work waits
+        tb = srv.submit(big)
+        ts = []
+        for i in range(24):
+            time.sleep(0.004)
+            ts.append(srv.submit(Request(small[0], small[1], I0, seed=100 + i, precision=precision)))
+        served = flat(srv.result(tb))
+        for t in ts:
+            srv.result(t)
+        st = srv.stats()
+    print(f"[serve/cut {precision}] {st}", flush=True)
+    with Batch([big]) as b:
+        b.launch()
+        b.wait()
+        ref = flat(b.read(0))
+    err = float(np.max(np.abs(ref - served)))
+    print(f"[serve/cut {precision}] max|Δ| = {err:.3e}", flush=True)
+    assert st["preemptions"] > 0 and st["resumed"] > 0
+    assert st["completed"] == 25 and st["failed"] == 0
+    assert err <= 1e-12 * float(np.max(np.abs(ref)))
