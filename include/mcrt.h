@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define MCRT_ABI_VERSION 1
+#define MCRT_ABI_VERSION 2
 
 /* Status codes.  MCRT_ERR_INVALID mirrors the reference's ArgumentError from
  * `validate` (src/geometries/plane.jl:52-62, src/geometries/sphere.jl:35-37). */
@@ -42,7 +42,7 @@ enum {
   MCRT_ERR_NO_DEVICE = -7
 };
 
-enum { MCRT_SCATTERER_RAYLEIGH = 0, MCRT_SCATTERER_MIE = 1 };
+enum { MCRT_SCATTERER_RAYLEIGH = 0, MCRT_SCATTERER_MIE = 1, MCRT_SCATTERER_SPLINE = 2 };
 enum { MCRT_GEOMETRY_PLANE = 0, MCRT_GEOMETRY_SPHERE = 1 };
 
 /* Arithmetic forms of the transport.
@@ -56,12 +56,24 @@ enum { MCRT_GEOMETRY_PLANE = 0, MCRT_GEOMETRY_SPHERE = 1 };
  *              the bin accumulation in FP64. */
 enum { MCRT_PRECISION_FP64_PLAIN = 0, MCRT_PRECISION_FP64 = 1, MCRT_PRECISION_MIXED = 2 };
 
-/* A scatterer: RayleighScatterer (src/scatterers/rayleigh.jl:3-6) or
- * MieScatterer (src/scatterers/mie.jl:3-15).  Tables are read only for Mie. */
+/* A scatterer: RayleighScatterer (src/scatterers/rayleigh.jl:3-6),
+ * MieScatterer (src/scatterers/mie.jl:3-15) or SplineScatterer
+ * (src/scatterers/spline.jl:3-15).  Tables are read for Mie and Spline only,
+ * F22 and F44 for Spline only (Mie's phase matrix has P22 = P11, P44 = P33).
+ *
+ * SplineScatterer: tables of one length n_theta on θ = range(0, π, n_theta),
+ * ω and kl as given, csrn = calculate_csrn of F11 (spline.jl:19-24;
+ * mcrt_calculate_csrn).  The reference's lookups (spline.jl:26-76) scale θ by
+ * `self.NSPL`, a name that is not defined there, so every call throws
+ * UndefVarError.  Here NSPL = n_theta - 1, the reading under which the lookup
+ * spans the table; everything else is kept as written, including the phase
+ * matrix's interpolation weights (msn+1-sn, sn-msn) on nodes msn, msn+1
+ * (spline.jl:35-50), which extrapolate one node step as Mie's amplitude
+ * lookup does (2·F[n_theta] at θ = π). */
 typedef struct mcrt_scatterer {
   int32_t kind;           /* MCRT_SCATTERER_* */
-  int32_t n_theta;        /* Mie: Nθ, length of F11..F34, S11, S22 */
-  int32_t random_points;  /* Mie: csrn holds random_points+1 values */
+  int32_t n_theta;        /* Mie, Spline: Nθ, length of F11..F44, S11, S22 */
+  int32_t random_points;  /* Mie, Spline: csrn holds random_points+1 values */
   int32_t reserved;
   double omega;           /* ω, single-scattering albedo */
   double kl;              /* kl, mean free path (also the CB phase scale) */
@@ -72,6 +84,8 @@ typedef struct mcrt_scatterer {
   const double* F34;
   const double* S11;      /* [2*n_theta] interleaved complex */
   const double* S22;
+  const double* F22;      /* Spline: [n_theta] */
+  const double* F44;      /* Spline: [n_theta] */
 } mcrt_scatterer;
 
 /* Layer (src/geometries/plane.jl:6-21).  d < 0 marks a semi-infinite layer
@@ -241,21 +255,29 @@ int mcrt_plan_create_range(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geo
                            uint64_t ray_begin, uint32_t instance, mcrt_plan** out);
 
 /* Launch shape of a cooperative plan (what the autotuner chooses): ray slots
- * per 512-thread workgroup (a multiple of 64; at most 256 mixed, 128 fp64) and
- * workgroups launched per CU (1..4).  0 selects the default: full slots (fewer
- * when N is too small to give every CU a workgroup) and as many workgroups
- * per CU as LDS allows, at most 2.  The plain form ignores it.  No reference
+ * per 512-thread workgroup (rays in flight, a multiple of 64; at most 256
+ * mixed, 128 fp64; the event records in LDS are sized by it), workgroups
+ * launched per CU (1..4), and where the CB amplitude table S is read (mixed
+ * form with CB and a tabulated scatterer: > 0 or 0 staged in LDS when it fits
+ * beside the tables and records, < 0 read through the caches from HBM; the
+ * fp64 form always reads it from HBM).  0 selects the defaults: as many slots
+ * as LDS allows (fewer when N is too small to give every CU a workgroup; S is
+ * given up before slots when tables are large) and as many workgroups per CU
+ * as LDS allows, at most 2.  The plain form ignores it.  No reference
  * counterpart. */
 typedef struct mcrt_launch_shape {
   int32_t slots;
   int32_t workgroups_per_cu;
+  int32_t stage_s;
+  int32_t reserved;
 } mcrt_launch_shape;
 
 int mcrt_plan_create_shaped(mcrt_ctx* ctx, const mcrt_config* cfg, const mcrt_geometry* geom,
                             const double I0[4], uint64_t seed, int32_t precision,
                             uint64_t ray_begin, uint32_t instance, const mcrt_launch_shape* shape,
                             mcrt_plan** out);
-/* The shape a plan was built with (workgroups: the launch's grid). */
+/* The shape a plan was built with (workgroups: the launch's grid; stage_s 1
+ * when S is staged in LDS, -1 when it is read from HBM). */
 int mcrt_plan_shape(mcrt_plan* plan, mcrt_launch_shape* shape, int32_t* workgroups);
 int mcrt_plan_launch(mcrt_plan* plan);   /* asynchronous on the context stream */
 int mcrt_plan_wait(mcrt_plan* plan);     /* blocks until the launch finished */
@@ -297,7 +319,12 @@ int mcrt_plan_read_sigma(mcrt_plan* plan, double A_sigma[5], double* Irt_sigma,
  *   SAMPLE      in (u) → cosθ drawn by inversion (rayleigh.jl:9-11, mie.jl:47-51)
  *   PHASE_TRANSPORT  in (cosθ) → the phase matrix as the transport's scatter
  *               evaluates it (utils.jl:298)
- *   KEPLER      in (M, e) → kepler_solver(M, e) (utils.jl:243-261) */
+ *   KEPLER      in (M, e) → kepler_solver(M, e) (utils.jl:243-261)
+ *   PHASE_DIAG  in (cosθ) → (P22, P44) of the peel-off's lookup, then (P22,
+ *               P44) as the transport evaluates them (equal to P11, P33 except
+ *               for a SplineScatterer, spline.jl:52-55)
+ * The table lookups use the scatterer of layer 1 (the plane's upper range and
+ * CB, or the sphere's). */
 enum {
   MCRT_EVAL_UPDATE_I = 0,
   MCRT_EVAL_UPDATE_E = 1,
@@ -307,7 +334,8 @@ enum {
   MCRT_EVAL_AMPLITUDE = 5,
   MCRT_EVAL_SAMPLE = 6,
   MCRT_EVAL_PHASE_TRANSPORT = 7,
-  MCRT_EVAL_KEPLER = 8
+  MCRT_EVAL_KEPLER = 8,
+  MCRT_EVAL_PHASE_DIAG = 9
 };
 int mcrt_plan_eval(mcrt_plan* plan, int32_t what, int64_t n, const double* in, double* out);
 

@@ -30,7 +30,7 @@ const PRECISION = Dict(:fp64_plain => Int32(0), :fp64 => Int32(1), :mixed => Int
 
 # ---------------------------------------------------------------- C structs
 struct McrtScatterer          # mcrt_scatterer
-    kind::Int32               # 0 Rayleigh, 1 Mie
+    kind::Int32               # 0 Rayleigh, 1 Mie, 2 Spline
     n_theta::Int32
     random_points::Int32
     reserved::Int32
@@ -43,6 +43,8 @@ struct McrtScatterer          # mcrt_scatterer
     F34::Ptr{Float64}
     S11::Ptr{Float64}         # ComplexF64 vectors seen as interleaved (re, im)
     S22::Ptr{Float64}
+    F22::Ptr{Float64}         # Spline only
+    F44::Ptr{Float64}
 end
 
 struct McrtLayer              # mcrt_layer
@@ -159,10 +161,11 @@ _f64(v) = v isa Vector{Float64} ? v : collect(Float64, v)
 """`mcrt_scatterer(s)` → (McrtScatterer, keep).  RayleighScatterer
 (rayleigh.jl:3-6) has no tables; MieScatterer (mie.jl:3-15) passes its
 F11..F34 and csrn vectors and its S11/S22 complex vectors as interleaved
-(re, im) Float64."""
+(re, im) Float64; SplineScatterer (spline.jl:3-15) passes F22 and F44 too, with
+Nθ = length(F11) and random_points = length(csrn) - 1."""
 function mcrt_scatterer(s::RayleighScatterer)
     return McrtScatterer(0, 0, 0, 0, s.ω, s.kl, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL, C_NULL,
-                         C_NULL), Any[]
+                         C_NULL, C_NULL, C_NULL), Any[]
 end
 
 function mcrt_scatterer(s::MieScatterer)
@@ -175,8 +178,22 @@ function mcrt_scatterer(s::MieScatterer)
         throw(ArgumentError("the Mie tables must hold Nθ values"))
     p(v) = Ptr{Float64}(pointer(v))
     c = McrtScatterer(1, s.Nθ, s.random_points, 0, s.ω, s.kl, p(csrn), p(F11), p(F12), p(F33),
-                      p(F34), p(S11), p(S22))
+                      p(F34), p(S11), p(S22), C_NULL, C_NULL)
     return c, Any[csrn, F11, F12, F33, F34, S11, S22]
+end
+
+function mcrt_scatterer(s::SplineScatterer)
+    csrn, F11, F12, F22 = _f64(s.csrn), _f64(s.F11), _f64(s.F12), _f64(s.F22)
+    F33, F34, F44 = _f64(s.F33), _f64(s.F34), _f64(s.F44)
+    S11 = collect(ComplexF64, s.S11)
+    S22 = collect(ComplexF64, s.S22)
+    N = length(F11)
+    all(length(v) == N for v in (F12, F22, F33, F34, F44, S11, S22)) ||
+        throw(ArgumentError("the Spline tables must have one length"))
+    p(v) = Ptr{Float64}(pointer(v))
+    c = McrtScatterer(2, N, length(csrn) - 1, 0, s.ω, s.kl, p(csrn), p(F11), p(F12), p(F33),
+                      p(F34), p(S11), p(S22), p(F22), p(F44))
+    return c, Any[csrn, F11, F12, F22, F33, F34, F44, S11, S22]
 end
 
 mcrt_scatterer(s) = throw(ArgumentError("libmcrt_hip has no form for $(typeof(s))"))

@@ -18,8 +18,12 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libmcrt_hip.so"
-SOURCES = ["mcrt_tables.cpp", "mcrt_api.cpp", "mcrt_batch.cpp", "mcrt_sched.cpp", "mcrt_serve.cpp",
-           "mcrt_plain.hip", "mcrt_coop.hip", "mcrt_static.hip"]
+# (source, object name, extra flags): mcrt_coop.hip is compiled once per part
+# (its MCRT_COOP_PART sections) so that its transport kernels build in parallel.
+SOURCES = [(src, src.rsplit(".", 1)[0], []) for src in (
+    "mcrt_tables.cpp", "mcrt_api.cpp", "mcrt_batch.cpp", "mcrt_sched.cpp", "mcrt_serve.cpp",
+    "mcrt_plain.hip", "mcrt_static.hip")] + [
+    ("mcrt_coop.hip", f"mcrt_coop_{part}", [f"-DMCRT_COOP_PART={part}"]) for part in range(1, 6)]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [
@@ -45,17 +49,20 @@ def _run(cmd: list[str], verbose: bool) -> None:
         sys.stderr.write(r.stdout + r.stderr)
 
 
-def build_library(verbose: bool = False, jobs: int = 4) -> Path:
+def build_library(verbose: bool = False, jobs: int = 0) -> Path:
     """Compile every source and link the shared library; returns its path."""
     if not Path(HIPCC).exists() and shutil.which("hipcc") is None:
         raise RuntimeError("hipcc not found: cannot build libmcrt_hip.so")
     BUILD.mkdir(exist_ok=True)
+    if jobs <= 0:
+        jobs = max(1, min(8, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                          else (os.cpu_count() or 4)))
     objs = []
     cmds = []
-    for src in SOURCES:
-        obj = BUILD / (src.rsplit(".", 1)[0] + ".o")
+    for src, name, flags in SOURCES:
+        obj = BUILD / (name + ".o")
         objs.append(obj)
-        cmds.append([HIPCC, *COMMON, "-c", str(CSRC / src), "-o", str(obj)])
+        cmds.append([HIPCC, *COMMON, *flags, "-c", str(CSRC / src), "-o", str(obj)])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB.with_suffix(".so.tmp")

@@ -115,7 +115,7 @@ using mcrt::fail;
                   "%s: %s", #expr, hipGetErrorString(e_));                              \
   } while (0)
 
-extern "C" const char* mcrt_version(void) { return "mcrt 0.1.0 (gfx950, ABI 1)"; }
+extern "C" const char* mcrt_version(void) { return "mcrt 0.2.0 (gfx950, ABI 2)"; }
 
 extern "C" const char* mcrt_last_error(const mcrt_ctx* ctx) {
   if (ctx) return ctx->err.c_str();
@@ -215,11 +215,14 @@ namespace {
 int validate_scatterer(mcrt_ctx* ctx, const mcrt_scatterer* s) {
   if (!s) return fail(ctx, MCRT_ERR_INVALID, "scatterer is NULL");
   if (s->kind == MCRT_SCATTERER_RAYLEIGH) return MCRT_OK;
-  if (s->kind != MCRT_SCATTERER_MIE) return fail(ctx, MCRT_ERR_INVALID, "unknown scatterer kind %d", s->kind);
+  if (s->kind != MCRT_SCATTERER_MIE && s->kind != MCRT_SCATTERER_SPLINE)
+    return fail(ctx, MCRT_ERR_INVALID, "unknown scatterer kind %d", s->kind);
+  const char* name = s->kind == MCRT_SCATTERER_MIE ? "Mie" : "Spline";
   if (s->n_theta < 2 || s->random_points < 1)
-    return fail(ctx, MCRT_ERR_INVALID, "Mie scatterer needs Nθ ≥ 2 and random_points ≥ 1");
-  if (!s->csrn || !s->F11 || !s->F12 || !s->F33 || !s->F34 || !s->S11 || !s->S22)
-    return fail(ctx, MCRT_ERR_INVALID, "Mie scatterer table pointer is NULL");
+    return fail(ctx, MCRT_ERR_INVALID, "%s scatterer needs Nθ ≥ 2 and random_points ≥ 1", name);
+  if (!s->csrn || !s->F11 || !s->F12 || !s->F33 || !s->F34 || !s->S11 || !s->S22 ||
+      (s->kind == MCRT_SCATTERER_SPLINE && (!s->F22 || !s->F44)))
+    return fail(ctx, MCRT_ERR_INVALID, "%s scatterer table pointer is NULL", name);
   return MCRT_OK;
 }
 
@@ -299,17 +302,24 @@ std::vector<const mcrt_scatterer*> scatterers_of(const mcrt_geometry* g, int* la
   return v;
 }
 
-// Interleave a scatterer's tables: F per node (F11, F12, F33, F34), S per node
-// (S11 re, S11 im, S22 re, S22 im).
+// Interleave a scatterer's tables: F per node (F11, F12, F33, F34) for Mie or
+// (F11, F12, F22, F33, F34, F44, 0, 0) for Spline, S per node (S11 re, S11 im,
+// S22 re, S22 im).
 template <typename T>
 void interleave(const mcrt_scatterer* s, std::vector<T>& F, std::vector<T>& S, std::vector<T>& csrn) {
-  int N = s->n_theta;
-  F.resize(4 * (size_t)N);
+  const int N = s->n_theta;
+  const int fs = mcrt::f_stride(s->kind);
+  F.assign((size_t)fs * N, T(0));
   S.resize(4 * (size_t)N);
   csrn.resize((size_t)s->random_points + 1);
   for (int i = 0; i < N; ++i) {
-    F[4 * i + 0] = (T)s->F11[i]; F[4 * i + 1] = (T)s->F12[i];
-    F[4 * i + 2] = (T)s->F33[i]; F[4 * i + 3] = (T)s->F34[i];
+    T* f = &F[(size_t)fs * i];
+    if (fs == 4) {
+      f[0] = (T)s->F11[i]; f[1] = (T)s->F12[i]; f[2] = (T)s->F33[i]; f[3] = (T)s->F34[i];
+    } else {
+      f[0] = (T)s->F11[i]; f[1] = (T)s->F12[i]; f[2] = (T)s->F22[i];
+      f[3] = (T)s->F33[i]; f[4] = (T)s->F34[i]; f[5] = (T)s->F44[i];
+    }
     S[4 * i + 0] = (T)s->S11[2 * i]; S[4 * i + 1] = (T)s->S11[2 * i + 1];
     S[4 * i + 2] = (T)s->S22[2 * i]; S[4 * i + 3] = (T)s->S22[2 * i + 1];
   }
@@ -386,10 +396,10 @@ int create_plain(mcrt_plan* P, const mcrt_config* c, const mcrt_geometry* g, con
   for (size_t i = 0; i < scas.size(); ++i) {
     const mcrt_scatterer* s = scas[i];
     mcrt::ScaView<double>& v = p.sca[i];
-    v.kind = s->kind == MCRT_SCATTERER_MIE ? 1 : 0;
+    v.kind = s->kind == MCRT_SCATTERER_MIE ? 1 : (s->kind == MCRT_SCATTERER_SPLINE ? 2 : 0);
     v.Nt = s->n_theta; v.rp = s->random_points;
     v.scale = 0; v.csrn = nullptr; v.F = nullptr; v.S = nullptr;
-    if (v.kind == 1) {
+    if (v.kind != 0) {
       std::vector<double> F, S, cs;
       interleave(s, F, S, cs);
       if (!(v.F = upload(P, F.data(), F.size(), &rc))) return rc;
@@ -435,7 +445,7 @@ extern "C" int mcrt_plan_create_shaped(mcrt_ctx* ctx, const mcrt_config* cfg,
                                        const mcrt_geometry* geom, const double I0[4], uint64_t seed,
                                        int32_t precision, uint64_t ray_begin, uint32_t instance,
                                        const mcrt_launch_shape* shape_in, mcrt_plan** out) {
-  const mcrt_launch_shape shape = shape_in ? *shape_in : mcrt_launch_shape{0, 0};
+  const mcrt_launch_shape shape = shape_in ? *shape_in : mcrt_launch_shape{0, 0, 0, 0};
   if (!ctx) return fail(nullptr, MCRT_ERR_INVALID, "context is NULL");
   if (!out || !I0) return fail(ctx, MCRT_ERR_INVALID, "plan_create: NULL argument");
   *out = nullptr;
@@ -504,6 +514,8 @@ extern "C" int mcrt_plan_shape(mcrt_plan* P, mcrt_launch_shape* shape, int32_t* 
   if (shape) {
     shape->slots = plain ? 0 : P->coop.slots;
     shape->workgroups_per_cu = plain ? 0 : P->coop.wg_per_cu;
+    shape->stage_s = plain ? 0 : (P->coop.stage_s ? 1 : -1);
+    shape->reserved = 0;
   }
   if (workgroups) *workgroups = plain ? P->plain_blocks : P->coop.blocks;
   return MCRT_OK;
@@ -616,7 +628,7 @@ extern "C" int mcrt_plan_read_sigma(mcrt_plan* P, double A_sigma[5], double* Irt
 extern "C" int mcrt_plan_eval(mcrt_plan* P, int32_t what, int64_t n, const double* in, double* out) {
   if (!P) return fail(nullptr, MCRT_ERR_INVALID, "plan is NULL");
   mcrt_ctx* ctx = P->ctx;
-  if (what < MCRT_EVAL_UPDATE_I || what > MCRT_EVAL_KEPLER)
+  if (what < MCRT_EVAL_UPDATE_I || what > MCRT_EVAL_PHASE_DIAG)
     return fail(ctx, MCRT_ERR_INVALID, "plan_eval: unknown function %d", what);
   if (n < 0 || (n > 0 && (!in || !out))) return fail(ctx, MCRT_ERR_INVALID, "plan_eval: bad buffers");
   if (n == 0) return MCRT_OK;

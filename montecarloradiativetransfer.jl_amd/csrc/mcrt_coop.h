@@ -33,7 +33,9 @@ struct CoopPlan {
   double* dIcb = nullptr;
   int64_t nrt = 0, ncb = 0;
   int geo_kind = 0;
-  bool mie = false;
+  int sk = 0;       // kernel kind: 0 Rayleigh, 1 Mie, 3 any (per-layer tables, Spline)
+  int sk0 = 0;      // kind of layer 1's scatterer (0 Rayleigh, 1 Mie, 2 Spline)
+  int stage_s = 0;  // S table staged in LDS
   bool cb = false;
   std::vector<const void*> table_refs;  // cached tables in use (TableCache)
 };
@@ -69,7 +71,7 @@ void table_cache_stats(TableCache* c, size_t* bytes, uint64_t* hits, uint64_t* m
 struct SolveDev {
   bool mixed = false;
   int geo_kind = 0;
-  bool mie = false, cb = false;
+  bool cb = false;
   int64_t N = 0, nout = 0, nrt = 0, ncb = 0;
   size_t lds_bytes = 0;
   void* params_dev = nullptr;

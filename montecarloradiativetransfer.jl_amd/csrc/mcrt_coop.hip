@@ -42,6 +42,16 @@
 #include "mcrt_coop.h"
 #include "mcrt_internal.h"
 
+// Translation units: build.py compiles this file once per part, so that the
+// transport kernels build in parallel: 1 the host code and the reduction
+// kernels, 2 the mixed form's plane kernels, 3 its sphere kernels, 4 its batch
+// and known-answer kernels, 5 the FP64 form's kernels.  0 (no definition):
+// everything in one unit.
+#ifndef MCRT_COOP_PART
+#define MCRT_COOP_PART 0
+#endif
+#define MCRT_PART(n) (MCRT_COOP_PART == 0 || MCRT_COOP_PART == (n))
+
 namespace mcrt {
 
 constexpr int kThreads = 512;
@@ -52,6 +62,12 @@ constexpr int kCbDx = 4;    // doubles of a CB event record (x_a - x_b, τab)
 constexpr int kBinW = 11;     // R words of a bin used by the evaluation
 constexpr int kBinStride = 12;  // R words per bin in the bin tables
 constexpr int kAcosTerms = 7;   // coefficients of the FP32 arccos polynomial (degree 6)
+
+// Scatterer kind of a kernel instantiation: Rayleigh, Mie or Spline tables for
+// every layer, or kAny: each table set's kind read at run time (a plane whose
+// layers hold different scatterers, or any SplineScatterer), dispatched once
+// per θ-range and per CB pass, outside the bin loops.
+constexpr int kRay = 0, kMie = 1, kSpl = 2, kAny = 3;
 
 // Per-precision launch shape: ray slots per workgroup and RT bins per
 // register chunk.
@@ -72,17 +88,23 @@ template <> struct Shape<double> {
   static constexpr double kPolar2 = 1e-24;  // the reference's x² + y² < 1e-24
 };
 
+// Lookup constants of one scatterer table: θ-index = acos(μ)·(Nθ - 1)/π.
+template <typename R> struct TabC {
+  R acs[kAcosTerms];  // FP32 arccos coefficients times (Nθ - 1)/π, highest degree first
+  R pis;              // π · (Nθ - 1)/π = Nθ - 1, the last node
+  R scale;            // (Nθ - 1)/π
+};
+
 template <typename R> struct CoopParams {
   int64_t N;
   int64_t maxscat;
   int slots;      // active ray slots per workgroup (multiple of 64, ≤ Shape<R>::kSlots)
+  int stage_s;    // the S table is staged in LDS (FP32 with CB; else read from HBM)
   int rt_cut;     // 0: no peel-off distance can exceed τ₀ (sphere with 2τ < τ₀)
   int cb_cut;     // 0: no CB distance can exceed τ₀ (sphere with 4τ < τ₀)
   R min_int;
   R tau0;
   R tau0k;      // tau0 · ExpK<R>::k
-  R acs[kAcosTerms];  // FP32 arccos coefficients times (Nθ_table - 1)/π, highest degree first
-  R pis;        // π · (Nθ_table - 1)/π = Nθ_table - 1, the last node
   int compute_cb;
   int geo_kind;  // 0 plane, 1 sphere
   int nl;
@@ -91,10 +113,20 @@ template <typename R> struct CoopParams {
   int isinf[kMaxLayers];
   double tau_d[kMaxLayers];
   double kl1;
-  // the one scatterer of the solve, in R (transport, bins) and FP64 (CB path state)
-  ScaView<R> sca;        // F and S padded with one zero node
+  // Table sets: 0 is layer 1's scatterer (the sphere's; the plane's upper
+  // θ-range, which only layer-1 events reach, and CB, which uses layer 1's
+  // scatterer, cpu.jl:50), 1 the last layer's (the plane's lower θ-range,
+  // which only last-layer events reach).  With one scatterer in every layer
+  // set 1 is set 0 (lo_shared).  In R (transport, bins) and FP64 (CB path
+  // state); F and S padded with one zero node.
+  ScaView<R> sca;
+  ScaView<R> sca_lo;
   ScaView<double> sca64;
-  R scale;               // (Nθ_table - 1)/π
+  TabC<R> tc[2];
+  int lo_shared;
+  int sk;  // the kernel kind running the solve (kRay, kMie, kAny)
+  // kAny: each layer's scatterer as the transport reads it (tables in HBM)
+  ScaView<R> lsca[kMaxLayers];
   // RT bins: ranges of θ rows, [lower (transmission), upper (reflection)]
   int Nt, Np, Ntb, Npb;
   int nranges;
@@ -166,18 +198,25 @@ MCRT_DEV void kahan(double* s, double* c, double x) {
 // weighted minimax polynomial of csrc/fit_acos.py (Remez exchange, |acos error|
 // ≤ 8.8e-8 rad, below the FP32 rounding of θ and of μ itself), its coefficients
 // pre-multiplied by (Nθ-1)/π, and acos(-a) = π - acos(a).  FP64: the library acos.
-template <typename R> MCRT_DEV R theta_index(const CoopParams<R>& p, R mu);
-template <> MCRT_DEV double theta_index(const CoopParams<double>& p, double mu) {
-  return ::acos(clamp1(mu)) * p.scale;
+// s2 ≥ 0 gives ρ² = 1 - μ² of the scattering plane: then 1 - a is formed as
+// ρ²/(1 + a), which keeps θ accurate within 0.02° of 0 and π where μ itself
+// rounds to ±1 in FP32 (the CB bins near exact backscattering, where the
+// reference's lookups jump at θ = π); s2 < 0 (a constant at the call site)
+// forms 1 - a from μ.
+template <typename R> MCRT_DEV R theta_index(const TabC<R>& t, R mu, R s2 = R(-1));
+template <> MCRT_DEV double theta_index(const TabC<double>& t, double mu, double) {
+  return ::acos(clamp1(mu)) * t.scale;
 }
-template <> MCRT_DEV float theta_index(const CoopParams<float>& p, float mu) {
+template <> MCRT_DEV float theta_index(const TabC<float>& t, float mu, float s2) {
   const float a = __builtin_fabsf(mu);
-  float c = p.acs[0];
+  float c = t.acs[0];
 #pragma unroll
-  for (int k = 1; k < kAcosTerms; ++k) c = __builtin_fmaf(c, a, p.acs[k]);
+  for (int k = 1; k < kAcosTerms; ++k) c = __builtin_fmaf(c, a, t.acs[k]);
   // 1 - a clamped to [0, 1] (|μ| can exceed 1 by rounding): the clamp bit of the subtraction
-  const float r = __builtin_amdgcn_sqrtf(__builtin_amdgcn_fmed3f(1.0f - a, 0.0f, 1.0f)) * c;
-  return mu < 0.0f ? p.pis - r : r;
+  const float om = s2 >= 0.0f ? s2 * __builtin_amdgcn_rcpf(1.0f + a)
+                              : __builtin_amdgcn_fmed3f(1.0f - a, 0.0f, 1.0f);
+  const float r = __builtin_amdgcn_sqrtf(om) * c;
+  return mu < 0.0f ? t.pis - r : r;
 }
 
 MCRT_DEV float fmax_r(float a, float b) { return __builtin_fmaxf(a, b); }
@@ -198,37 +237,63 @@ MCRT_DEV int node_of(double sn, double imax, double& f) {
   return i;
 }
 
-// Phase matrix (P11, P12, P33, P34) at cosθ = mu: Rayleigh (rayleigh.jl:12-17)
-// or a lookup in the padded LDS table F, linear in θ between nodes i0 and
-// i0 + 1 (mie.jl:53-71).
-template <typename R, bool MIE>
-MCRT_DEV void phase_tab(const CoopParams<R>& p, const R* F, R mu, R P[4]) {
-  if constexpr (!MIE) {
+// Phase matrix (P11, P12, P22, P33, P34, P44) of a bin lookup.
+template <typename R> struct Ph { R p11, p12, p22, p33, p34, p44; };
+
+// Phase matrix at cosθ = mu: Rayleigh (rayleigh.jl:12-17), a lookup in the
+// padded LDS table F linear in θ between nodes i0 and i0 + 1 (Mie,
+// mie.jl:53-71, P22 = P11, P44 = P33), or the Spline lookup (spline.jl:26-58)
+// with its weights (msn+1-sn, sn-msn) = (2-f, f-1) on nodes i0, i0 + 1 and
+// six elements per node (the zero pad node gives 2·F at θ = π as written).
+template <typename R, int SK>
+MCRT_DEV Ph<R> phase_tab(const TabC<R>& t, const R* F, R mu, R s2 = R(-1)) {
+  Ph<R> P;
+  if constexpr (SK == kRay) {
     R m2 = mu * mu;
-    P[0] = R(0.75) * (R(1) + m2);
-    P[1] = R(-0.75) * (R(1) - m2);
-    P[2] = R(1.5) * mu;
-    P[3] = R(0);
-  } else {
+    P.p11 = R(0.75) * (R(1) + m2);
+    P.p12 = R(-0.75) * (R(1) - m2);
+    P.p33 = R(1.5) * mu;
+    P.p34 = R(0);
+    P.p22 = P.p11;
+    P.p44 = P.p33;
+  } else if constexpr (SK == kMie) {
     R f;
-    const int i0 = node_of(theta_index(p, mu), p.pis, f);
+    const int i0 = node_of(theta_index(t, mu, s2), t.pis, f);
     const R* a = F + 4 * i0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) P[c] = a[c] + f * (a[4 + c] - a[c]);
+    P.p11 = a[0] + f * (a[4] - a[0]);
+    P.p12 = a[1] + f * (a[5] - a[1]);
+    P.p33 = a[2] + f * (a[6] - a[2]);
+    P.p34 = a[3] + f * (a[7] - a[3]);
+    P.p22 = P.p11;
+    P.p44 = P.p33;
+  } else {
+    static_assert(SK == kSpl, "phase_tab: a table kind");
+    R f;
+    const int i0 = node_of(theta_index(t, mu, s2), t.pis, f);
+    const R g = R(1) - f;
+    const R* a = F + 8 * i0;
+    P.p11 = a[0] + g * (a[0] - a[8]);
+    P.p12 = a[1] + g * (a[1] - a[9]);
+    P.p22 = a[2] + g * (a[2] - a[10]);
+    P.p33 = a[3] + g * (a[3] - a[11]);
+    P.p34 = a[4] + g * (a[4] - a[12]);
+    P.p44 = a[5] + g * (a[5] - a[13]);
   }
+  return P;
 }
 
-// Amplitude matrix diag(S11, S22) (rayleigh.jl:20-23; mie.jl:76-92 with its
-// weights (msn+1-sn, sn-msn) = (2-f, f-1) on nodes i0, i0 + 1: the zero pad
-// node reproduces the skipped term at θ = π).
-template <typename R, bool MIE>
-MCRT_DEV void amp_tab(const CoopParams<R>& p, const R* S, R mu, Cx<R>& S11, Cx<R>& S22) {
-  if constexpr (!MIE) {
+// Amplitude matrix diag(S11, S22) (rayleigh.jl:20-23; mie.jl:76-92 and
+// spline.jl:60-76, the same lookup, with its weights (msn+1-sn, sn-msn) =
+// (2-f, f-1) on nodes i0, i0 + 1: the zero pad node reproduces the skipped
+// term at θ = π).
+template <typename R, int SK>
+MCRT_DEV void amp_tab(const TabC<R>& t, const R* S, R mu, Cx<R>& S11, Cx<R>& S22, R s2 = R(-1)) {
+  if constexpr (SK == kRay) {
     S11 = {R(0), R(1)};
     S22 = {mu * R(0), mu};
   } else {
     R f;
-    const int i0 = node_of(theta_index(p, mu), p.pis, f);
+    const int i0 = node_of(theta_index(t, mu, s2), t.pis, f);
     const R g = R(1) - f;
     const R* a = S + 4 * i0;
     S11 = {a[0] + g * (a[0] - a[4]), a[1] + g * (a[1] - a[5])};
@@ -283,8 +348,9 @@ template <typename R> MCRT_DEV void frame_matrix(Fr<R> f, R m[8]) {
 // θₖ ∈ {0, π}; here ρ² < Shape<R>::kPolar2) is taken by a wave-uniform test,
 // since exact alignment of a bin with an event direction is rare (a
 // divergent branch that is skipped when no lane of the wave takes it).
-// c2 = rx² - ry² = ρ²·cos2ϕₖ and r2 = ρ² are kept for the Stokes rotation.
-template <typename R> struct Plane { R mu, rx, ry, pn, w, c2, r2, inv; };
+// c2 = rx² - ry² = ρ²·cos2ϕₖ and r2 = ρ² are kept for the Stokes rotation, s2
+// (ρ² as computed, -1 in the polar case) for the lookups of the CB bins.
+template <typename R> struct Plane { R mu, rx, ry, pn, w, c2, r2, inv, s2; };
 
 MCRT_DEV float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 MCRT_DEV double fma_r(double a, double b, double c) { return ::fma(a, b, c); }
@@ -300,6 +366,7 @@ MCRT_DEV R plane_raw(Plane<R>& q, const R* m, R kx, R ky, R kz, R e1, R e2, R g0
   const R ry2 = q.ry * q.ry;
   q.c2 = fma_r(q.rx, q.rx, -ry2);
   q.r2 = fma_r(q.rx, q.rx, ry2);
+  q.s2 = q.r2;
   return q.r2;
 }
 
@@ -313,6 +380,7 @@ MCRT_DEV R plane_polar(Plane<R>& q, const R* m, R e1, R e2) {
   q.w = q.mu > R(0) ? -u : (q.mu < R(0) ? u : R(0));
   q.c2 = R(1);
   q.r2 = R(1);
+  q.s2 = R(-1);  // the lookup takes θ from μ = ±1, as the reference's polar case
   return R(1);
 }
 
@@ -329,19 +397,18 @@ MCRT_DEV Plane<R> scat_plane(const R* m, R kx, R ky, R kz, R e1, R e2, R g0, R g
 // (utils.jl:91-112) = R(ψ)·P(μ)·R(-ϕₖ)·I on the scattering plane q.  I is the
 // event's Stokes vector, J = (2·I₂, -2·I₃).  Returns (o0, o1·ρ², o2·ρ², o3):
 // the middle pair still carries the factor ρ² (multiply by q.inv).
-template <typename R, bool MIE>
-MCRT_DEV void peel_I(const CoopParams<R>& p, const R* F, const R* I, const R* J,
+template <typename R, int SK, bool ACC = false>
+MCRT_DEV void peel_I(const TabC<R>& t, const R* F, const R* I, const R* J,
                      const Plane<R>& q, R o[4]) {
-  R P[4];
-  phase_tab<R, MIE>(p, F, q.mu, P);
+  const Ph<R> P = phase_tab<R, SK>(t, F, q.mu, ACC ? q.s2 : R(-1));
   const R c2 = q.c2;                      // ρ²·cos2ϕₖ
   const R sh = q.rx * q.ry;                // -ρ²·sin2ϕₖ / 2
   const R i1 = (c2 * I[1] + sh * J[1]) * q.inv;
   const R i2 = (sh * J[0] + c2 * I[2]) * q.inv;
-  const R a0 = P[0] * I[0] + P[1] * i1;
-  const R a1 = P[1] * I[0] + P[0] * i1;
-  const R a2 = P[2] * i2 + P[3] * I[3];
-  const R a3 = -P[3] * i2 + P[2] * I[3];
+  const R a0 = P.p11 * I[0] + P.p12 * i1;
+  const R a1 = P.p12 * I[0] + P.p22 * i1;
+  const R a2 = P.p33 * i2 + P.p34 * I[3];
+  const R a3 = -P.p34 * i2 + P.p44 * I[3];
   // ρ²·cos2ψ = pn² - w² = ρ² - 2w² (pn² + w² = ρ²: cos²ψ + sin²ψ = 1), ρ²·sin2ψ = 2w·pn
   const R w2 = q.w + q.w;
   const R c2p = fma_r(-w2, q.w, q.r2);
@@ -355,11 +422,12 @@ MCRT_DEV void peel_I(const CoopParams<R>& p, const R* F, const R* I, const R* J,
 // update_E (utils.jl:170-192) on the scattering plane q: E ← Rψ·S(μ)·Rϕₖ·E,
 // returned times ρ²·sc (sc = q.inv gives update_E itself); the real factor sc
 // (e.g. an attenuation) rides on the two coefficients of the ψ rotation.
-template <typename R, bool MIE>
-MCRT_DEV void peel_E(const CoopParams<R>& p, const R* S, const Plane<R>& q, R sc, Cx<R>& E1,
+// ACC: the lookup's θ from ρ² (theta_index).
+template <typename R, int SK, bool ACC = false>
+MCRT_DEV void peel_E(const TabC<R>& t, const R* S, const Plane<R>& q, R sc, Cx<R>& E1,
                      Cx<R>& E2) {
   Cx<R> S11, S22;
-  amp_tab<R, MIE>(p, S, q.mu, S11, S22);
+  amp_tab<R, SK>(t, S, q.mu, S11, S22, ACC ? q.s2 : R(-1));
   const Cx<R> f1 = S11 * (E1 * q.rx + E2 * q.ry);
   const Cx<R> f2 = S22 * (E2 * q.rx + E1 * -q.ry);
   const R pn = q.pn * sc, w = q.w * sc;
@@ -456,9 +524,9 @@ template <typename R> MCRT_DEV Slot<R>* slot_block(void* base, int block) {
 // RT record: m[0..7], I[8..11], J[12..13], K·x[14..16], [17] K·τ of the layer
 // (plane) or K²·(τ² - x·x) (sphere), [18] 1 or 0: near the upper (first) boundary,
 // [19] 1 or 0: near the lower (last) boundary; K = ExpK<R>::k.
-template <int CH, typename R, int GEO, bool MIE, bool CUT>
-MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc, double* slab,
-                      int E, int tid, int lane, int wave, int jb, int nrow, int nb,
+template <int CH, typename R, int GEO, int SK, bool CUT>
+MCRT_DEV void rt_pass(const CoopParams<R>& p, const TabC<R>& t, const R* F, const R* rtrec, R* esc,
+                      double* slab, int E, int tid, int lane, int wave, int jb, int nrow, int nb,
                       int c0, int fw, int slot2) {
   R bk[CH][kBinW];
   int bidx[CH];
@@ -511,7 +579,7 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
         Plane<R>& q = qs[i];
         q.inv = M<R>::div(R(1), r2[i]);
         R o[4];
-        peel_I<R, MIE>(p, F, rec + 8, rec + 12, q, o);
+        peel_I<R, SK>(t, F, rec + 8, rec + 12, q, o);
         R t;  // K·distance_to_boundary
         if (GEO == 0) {  // plane.jl:97-105
           t = (bk[i][2] > R(0) ? rec[16] : rec[16] + rec[17]) * -bk[i][9];
@@ -546,46 +614,70 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc
   }
 }
 
-// All θ-ranges of the peel-off: full register chunks of Shape<R>::kChunk bins,
-// then the remainder in passes of 4, 2 and 1.
-template <typename R, int GEO, bool MIE, bool CUT>
-MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F, const R* rtrec, R* esc, double* slab,
-                        int E, int tid, int lane, int wave) {
+// One θ-range of the peel-off with the table set (t, F): full register chunks
+// of Shape<R>::kChunk bins, then the remainder in passes of 4, 2 and 1.
+template <typename R, int GEO, int SK, bool CUT>
+MCRT_DEV void rt_range(const CoopParams<R>& p, const TabC<R>& t, const R* F, const R* rtrec, R* esc,
+                       double* slab, int E, int tid, int lane, int wave, int rg) {
   constexpr int CH = Shape<R>::kChunk;
+  const int jb = p.range_b[rg], je = p.range_e[rg];
+  const int nrow = je - jb;
+  const int nb = nrow * p.Np;  // bins of this range
+  const int per = (nb + kThreads - 1) / kThreads;
+  const int fw = p.range_kind[rg] == 1 ? 18 : 19;  // record word of the near flag
+  const int slot2 = p.range_kind[rg] == 1 ? 0 : 1;
+  int c0 = 0;
+  // with chunks of 8, a remainder of one bin joins the last chunk (a pass of
+  // 9: e.g. 17 bins per thread at 180 × 48 run as 9 + 8, and 9 as one pass),
+  // since every pass pays the per-event overhead (record reads, reduction)
+  if constexpr (CH == 8) {
+    if (per % 8 == 1 && per >= 9) {
+      rt_pass<9, R, GEO, SK, CUT>(p, t, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow, nb, c0,
+                                  fw, slot2);
+      c0 += 9;
+    }
+  }
+  for (; c0 + CH <= per; c0 += CH)
+    rt_pass<CH, R, GEO, SK, CUT>(p, t, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow, nb, c0,
+                                 fw, slot2);
+  if (CH > 4 && c0 + 4 <= per) {
+    rt_pass<4, R, GEO, SK, CUT>(p, t, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow, nb, c0,
+                                fw, slot2);
+    c0 += 4;
+  }
+  if (c0 + 2 <= per) {
+    rt_pass<2, R, GEO, SK, CUT>(p, t, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow, nb, c0,
+                                fw, slot2);
+    c0 += 2;
+  }
+  if (c0 < per)
+    rt_pass<1, R, GEO, SK, CUT>(p, t, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow, nb, c0,
+                                fw, slot2);
+}
+
+// All θ-ranges of the peel-off.  The upper range (reflection; the sphere's
+// only range) uses table set 0 (F0), the plane's lower range table set 1 (F1).
+// kAny dispatches on the set's kind once per range: a sphere's set is a
+// Spline (Rayleigh and Mie spheres have kernels of their own), a plane's any.
+template <typename R, int GEO, int SK, bool CUT>
+MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F0, const R* F1, const R* rtrec, R* esc,
+                        double* slab, int E, int tid, int lane, int wave) {
   for (int rg = 0; rg < p.nranges; ++rg) {
-    const int jb = p.range_b[rg], je = p.range_e[rg];
-    const int nrow = je - jb;
-    const int nb = nrow * p.Np;  // bins of this range
-    const int per = (nb + kThreads - 1) / kThreads;
-    const int fw = p.range_kind[rg] == 1 ? 18 : 19;  // record word of the near flag
-    const int slot2 = p.range_kind[rg] == 1 ? 0 : 1;
-    int c0 = 0;
-    // with chunks of 8, a remainder of one bin joins the last chunk (a pass of
-    // 9: e.g. 17 bins per thread at 180 × 48 run as 9 + 8, and 9 as one pass),
-    // since every pass pays the per-event overhead (record reads, reduction)
-    if constexpr (CH == 8) {
-      if (per % 8 == 1 && per >= 9) {
-        rt_pass<9, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow, nb,
-                                     c0, fw, slot2);
-        c0 += 9;
+    if constexpr (SK != kAny) {
+      rt_range<R, GEO, SK, CUT>(p, p.tc[0], F0, rtrec, esc, slab, E, tid, lane, wave, rg);
+    } else {
+      const int set = p.range_kind[rg] == 1 ? 0 : 1;
+      const int kind = set ? p.sca_lo.kind : p.sca.kind;
+      const R* F = set ? F1 : F0;
+      if (kind == kSpl) {
+        rt_range<R, GEO, kSpl, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);
+      } else if constexpr (GEO == 0) {
+        if (kind == kMie)
+          rt_range<R, GEO, kMie, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);
+        else
+          rt_range<R, GEO, kRay, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);
       }
     }
-    for (; c0 + CH <= per; c0 += CH)
-      rt_pass<CH, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb,
-                                            nrow, nb, c0, fw, slot2);
-    if (CH > 4 && c0 + 4 <= per) {
-      rt_pass<4, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow,
-                                           nb, c0, fw, slot2);
-      c0 += 4;
-    }
-    if (c0 + 2 <= per) {
-      rt_pass<2, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow,
-                                           nb, c0, fw, slot2);
-      c0 += 2;
-    }
-    if (c0 < per)
-      rt_pass<1, R, GEO, MIE, CUT>(p, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow,
-                                           nb, c0, fw, slot2);
   }
 }
 
@@ -604,9 +696,10 @@ MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F, const R* rtrec, R* e
 // scaled for exp2 as in the peel-off.  CUT = false when no CB distance can
 // exceed τ₀ (a sphere with 4τ < τ₀: τ_a ≤ 2τ and |τ_b + τ_ab| ≤ 4τ), where both
 // cut-off tests (cpu.jl:97, 104) always pass.
-template <typename R, int GEO, bool MIE, bool ATOMIC, bool CUT>
+template <typename R, int GEO, int SK, bool ATOMIC, bool CUT>
 MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* cbrec,
                       const double* cbdx, double* slab, int E, int tid) {
+  const TabC<R>& t = p.tc[0];
   const int nb = p.Ntb * p.Npb;
   // split only iterations with enough events to share (uniform over the workgroup)
   const int G = E >= 8 * p.cb_groups ? p.cb_groups : 1;
@@ -648,7 +741,7 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
       // s0 (I, V) and s12 (Q, U) that scale it into the accumulation
       R o[4], s0, s12;
       if (kind == 1) {  // n = 1 (cpu.jl:131-158)
-        peel_I<R, MIE>(p, F, cr, cr + 32, qa, o);
+        peel_I<R, SK, true>(t, F, cr, cr + 32, qa, o);
         s0 = ExpK<R>::exp_neg(ta);
         s12 = s0 * qa.inv;
       } else {          // n > 1 (cpu.jl:83-127)
@@ -668,8 +761,8 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
         // the attenuations e^{-τ/2} and the 1/ρ² of each plane ride on peel_E's ψ rotation
         const R na = ExpK<R>::exp_neg(R(0.5) * ta) * qa.inv;
         const R nb_ = bok ? ExpK<R>::exp_neg(R(0.5) * (tb + tab)) * qb.inv : R(0);
-        peel_E<R, MIE>(p, S, qa, na, a1, a2);
-        peel_E<R, MIE>(p, S, qb, nb_, b1, b2);
+        peel_E<R, SK, true>(t, S, qa, na, a1, a2);
+        peel_E<R, SK, true>(t, S, qb, nb_, b1, b2);
         // the phase kl·(k_i + k_f)·(x_a - x_b) (cpu.jl:94), kl and the unit folded
         // into the per-bin k_i + k_f
         const double dphi = kikx * dx[0] + kiky * dx[1] + kikz * dx[2];
@@ -705,45 +798,224 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
   }
 }
 
-// LDS blocks of one workgroup for the solve p; every block is a multiple of 16
-// bytes and lds_plan computes the same total.  F first: its LDS address is then
+// LDS blocks of one workgroup for the solve p, as byte offsets from the
+// dynamic LDS base (kNoLds: absent); every block is a multiple of 16 bytes and
+// the host sizes the launch with the same function.  The event records are
+// sized by the solve's active slots.  Set 0's F first: its LDS address is then
 // the lookup offset itself.
+constexpr size_t kNoLds = ~size_t(0);
+struct LdsOff { size_t F, rtrec, cbrec, cbdx, esc, wcnt, C, S, F1, end; };
+template <typename R, int SK, bool CB>
+MCRT_HD LdsOff lds_offsets(const CoopParams<R>& p) {
+  LdsOff o;
+  size_t q = 0;
+  auto take = [&q](size_t bytes) { const size_t r = q; q += (bytes + 15) & ~size_t(15); return r; };
+  const int k0 = SK == kAny ? p.sca.kind : SK;
+  const bool tab0 = k0 != kRay;
+  const size_t n = (size_t)p.slots;
+  o.F = tab0 ? take(sizeof(R) * f_stride(k0) * (size_t)(p.sca.Nt + 1)) : kNoLds;
+  o.rtrec = take(sizeof(R) * kRtRec * n);
+  o.cbrec = CB ? take(sizeof(R) * kCbRec * n) : kNoLds;
+  o.cbdx = CB ? take(sizeof(double) * kCbDx * n) : kNoLds;
+  o.esc = take(sizeof(R) * n * kWaves * 2);
+  o.wcnt = take(sizeof(int) * 32);
+  // csrn of the transport's sampling (kAny samples from the tables in HBM)
+  o.C = (SK != kAny && tab0) ? take(sizeof(R) * (size_t)(p.sca.rp + 1)) : kNoLds;
+  o.S = (CB && tab0 && p.stage_s) ? take(sizeof(R) * 4 * (size_t)(p.sca.Nt + 1)) : kNoLds;
+  o.F1 = (SK == kAny && !p.lo_shared && p.sca_lo.kind != kRay)
+             ? take(sizeof(R) * f_stride(p.sca_lo.kind) * (size_t)(p.sca_lo.Nt + 1))
+             : kNoLds;
+  o.end = q;
+  return o;
+}
+
 template <typename R> struct Lds {
-  R* F;
+  R* F;       // set 0's F table
   R* rtrec;
   R* cbrec;
   double* cbdx;
   R* esc;
   int* wcnt;
-  R* C;
-  R* S;
+  R* C;       // set 0's csrn
+  R* S;       // set 0's S table (NULL: read from HBM)
+  R* F1;      // set 1's F table when it is not set 0's
 };
-template <typename R, bool MIE, bool CB>
+template <typename R, int SK, bool CB>
 MCRT_DEV Lds<R> lds_layout(const CoopParams<R>& p, unsigned char* q) {
-  constexpr int SLOTS = Shape<R>::kSlots;
-  constexpr bool STAGE_S = Shape<R>::kStageS && MIE && CB;
-  auto take = [&q](size_t bytes) { unsigned char* r = q; q += (bytes + 15) & ~size_t(15); return r; };
+  const LdsOff o = lds_offsets<R, SK, CB>(p);
+  auto at = [q](size_t off) { return off == kNoLds ? nullptr : q + off; };
   Lds<R> L;
-  L.F = MIE ? (R*)take(sizeof(R) * 4 * (p.sca.Nt + 1)) : nullptr;
-  L.rtrec = (R*)take(sizeof(R) * kRtRec * SLOTS);
-  L.cbrec = CB ? (R*)take(sizeof(R) * kCbRec * SLOTS) : nullptr;
-  L.cbdx = CB ? (double*)take(sizeof(double) * kCbDx * SLOTS) : nullptr;
-  L.esc = (R*)take(sizeof(R) * SLOTS * kWaves * 2);
-  L.wcnt = (int*)take(sizeof(int) * 32);
-  L.C = MIE ? (R*)take(sizeof(R) * (p.sca.rp + 1)) : nullptr;
-  L.S = STAGE_S ? (R*)take(sizeof(R) * 4 * (p.sca.Nt + 1)) : nullptr;
+  L.F = (R*)at(o.F);
+  L.rtrec = (R*)at(o.rtrec);
+  L.cbrec = (R*)at(o.cbrec);
+  L.cbdx = (double*)at(o.cbdx);
+  L.esc = (R*)at(o.esc);
+  L.wcnt = (int*)at(o.wcnt);
+  L.C = (R*)at(o.C);
+  L.S = (R*)at(o.S);
+  L.F1 = (R*)at(o.F1);
   return L;
 }
 
-// Copy the solve's Mie tables into LDS (the caller synchronises afterwards).
-template <typename R, bool MIE, bool CB>
+// Copy the solve's tables into LDS (the caller synchronises afterwards).
+template <typename R>
 MCRT_DEV void stage_tables(const CoopParams<R>& p, const Lds<R>& L, int tid) {
-  if constexpr (MIE) {
-    for (int i = tid; i < 4 * (p.sca.Nt + 1); i += kThreads) L.F[i] = p.sca.F[i];
+  if (L.F)
+    for (int i = tid; i < f_stride(p.sca.kind) * (p.sca.Nt + 1); i += kThreads) L.F[i] = p.sca.F[i];
+  if (L.C)
     for (int i = tid; i <= p.sca.rp; i += kThreads) L.C[i] = p.sca.csrn[i];
-  }
-  if constexpr (Shape<R>::kStageS && MIE && CB) {
+  if (L.S)
     for (int i = tid; i < 4 * (p.sca.Nt + 1); i += kThreads) L.S[i] = p.sca.S[i];
+  if (L.F1)
+    for (int i = tid; i < f_stride(p.sca_lo.kind) * (p.sca_lo.Nt + 1); i += kThreads)
+      L.F1[i] = p.sca_lo.F[i];
+}
+
+// The CB pass with set 0's tables (cpu.jl:50 takes layer 1's scatterer);
+// kAny dispatches on its kind.
+template <typename R, int GEO, int SK, bool ATOMIC>
+MCRT_DEV void cb_all(const CoopParams<R>& p, const R* F, const R* S, const R* cbrec,
+                     const double* cbdx, double* slab, int E, int tid) {
+  if constexpr (SK != kAny) {
+    if (GEO == 1 && !p.cb_cut) cb_pass<R, GEO, SK, ATOMIC, false>(p, F, S, cbrec, cbdx, slab, E, tid);
+    else cb_pass<R, GEO, SK, ATOMIC, true>(p, F, S, cbrec, cbdx, slab, E, tid);
+  } else {
+    if (p.sca.kind == kSpl) {
+      cb_all<R, GEO, kSpl, ATOMIC>(p, F, S, cbrec, cbdx, slab, E, tid);
+    } else if constexpr (GEO == 0) {
+      if (p.sca.kind == kMie) cb_all<R, GEO, kMie, ATOMIC>(p, F, S, cbrec, cbdx, slab, E, tid);
+      else cb_all<R, GEO, kRay, ATOMIC>(p, F, S, cbrec, cbdx, slab, E, tid);
+    }
+  }
+}
+
+// The CB path state and record of one slot's event: the slot sp holds the
+// event's direction k (arriving at x), position x, layer idx, Stokes vector I
+// after absorption and event number n (the transport's scatter has written
+// only kp, xp, idxp, Ip beside them); rec is its RT record (near flags).
+//  * forward_E advanced one step (utils.jl:205-213) and the reversed path of
+//    backward_E as a normalised running 2×2 product (utils.jl:230-238);
+//  * collect_cb's set-up for n > 1 (cpu.jl:53-81) or the n = 1 record
+//    (cpu.jl:131-158), or an empty record (kind 0).
+// CB record: [0..7] the scaled forward and reversed fields (n > 1) or I and
+// [32..33] J (n = 1); [8..15] frame matrix of the event direction, [16..23]
+// of -k₂; [24..26] K·x, [27..29] K·x₁; [30], [31] K·τ of the layer at x and x₁
+// (plane) or K²·(τ² - x·x) and K²·(τ² - x₁·x₁) (sphere); [39] kind;
+// dx: x - x₁ and K·τab in FP64.
+template <typename R, int GEO>
+MCRT_DEV void cb_event(const CoopParams<R>& p, Slot<R>* sp, const R* rec, R* cr, double* dx) {
+  const int64_t n = sp->n;
+  const V3<R> x = sp->x;
+  const int idx = sp->idx;
+  if (n == 1) {
+    sp->x1d = {(double)x.x, (double)x.y, (double)x.z};
+    sp->idx1 = idx;
+  }
+  const V3<R> k = sp->k;
+  const V3<double> kd = {(double)k.x, (double)k.y, (double)k.z};
+  Cx<double> Ea1 = sp->Ea1, Ea2 = sp->Ea2;
+  double enorma = sp->enorma;
+  if (n >= 2) {  // forward_E advanced by one step (utils.jl:205-213)
+    update_E(p.sca64, Ea1, Ea2, kd, sp->ksf);
+    double a = hypot(Ea1.re, Ea1.im), b = hypot(Ea2.re, Ea2.im);
+    double norm2 = sqrt(a * a + b * b);
+    double r = p.norm0 / norm2;
+    Ea1 = Ea1 * r;
+    Ea2 = Ea2 * r;
+    enorma += log(p.norm0 / norm2);
+    sp->Ea1 = Ea1;
+    sp->Ea2 = Ea2;
+    sp->enorma = enorma;
+    sp->ksf = sph(kd);
+  }
+  // reversed path as a running product (utils.jl:230-238)
+  Lmap<double> Q = {{1, 0}, {0, 0}, {0, 0}, {1, 0}};  // (used for n > 1 only)
+  double lq = 0.0;
+  Fr<double> kbs = {1, 0, 1, 0};
+  if (n == 2) {
+    Q = {{1, 0}, {0, 0}, {0, 0}, {1, 0}};
+    lq = 0.0;
+    kbs = sph(-kd);
+    sp->Q = Q;
+    sp->lq = lq;
+    sp->kbs = kbs;
+  } else if (n >= 3) {
+    Lmap<double> L = update_E_map(p.sca64, -sp->kprev, sph(-kd));
+    Q = mul(sp->Q, L);
+    lq = sp->lq;
+    lmap_normalise(Q, lq);
+    kbs = sp->kbs;
+    sp->Q = Q;
+    sp->lq = lq;
+  }
+  sp->kprev = kd;
+
+  const int cb_on = p.compute_cb && (rec[18] != R(0) || rec[19] != R(0));
+  const Fr<double> kas = sph(kd);
+  const V3<double> xa = {(double)x.x, (double)x.y, (double)x.z};
+  const V3<double> kf0 = {p.kf0[0], p.kf0[1], p.kf0[2]};
+  const R K = ExpK<R>::k;
+  if (cb_on && n > 1) {
+    // collect_cb n > 1 set-up (cpu.jl:53-81) from the O(1) state
+    const V3<double> x1d = sp->x1d;
+    const int idx1 = sp->idx1;
+    double ta = dist64(p, xa, kf0, idx);
+    double tb = dist64(p, x1d, kf0, idx1);
+    const R* I = sp->I;
+    double Id[4] = {(double)I[0], (double)I[1], (double)I[2], (double)I[3]};
+    double Iaf[4];
+    update_I(p.sca64, Id, kf0, kas, Iaf);
+    // reversed path: Q · L(-k_n | frame₀) · E_i, scaled by exp(lq + enorma)
+    Fr<double> f0 = {p.f0[0], p.f0[1], p.f0[2], p.f0[3]};
+    Lmap<double> L0 = update_E_map(p.sca64, -kd, f0);
+    Cx<double> Ei1 = {p.Ei[0], p.Ei[1]}, Ei2 = {p.Ei[2], p.Ei[3]};
+    Cx<double> u1 = L0.a * Ei1 + L0.b * Ei2, u2 = L0.c * Ei1 + L0.d * Ei2;
+    Cx<double> v1 = Q.a * u1 + Q.b * u2, v2 = Q.c * u1 + Q.d * u2;
+    double sc = exp(lq + enorma);
+    Cx<double> Eb1 = v1 * sc, Eb2 = v2 * sc;
+    Cx<double> Eaf1 = Ea1, Eaf2 = Ea2;
+    update_E(p.sca64, Eaf1, Eaf2, kf0, kas);
+    // raa (cpu.jl:68-69); zero where rounding leaves Iaf₁ ≤ 0 or the
+    // field is zero (the reference would raise DomainError or give NaN)
+    const double eaf2 = Eaf1.re * Eaf1.re + Eaf1.im * Eaf1.im + Eaf2.re * Eaf2.re +
+                        Eaf2.im * Eaf2.im;
+    const double raa = (Iaf[0] > 0 && eaf2 > 0) ? sqrt(Iaf[0] / eaf2) : 0.0;
+    cr[0] = (R)(Ea1.re * raa); cr[1] = (R)(Ea1.im * raa);
+    cr[2] = (R)(Ea2.re * raa); cr[3] = (R)(Ea2.im * raa);
+    cr[4] = (R)(Eb1.re * raa); cr[5] = (R)(Eb1.im * raa);
+    cr[6] = (R)(Eb2.re * raa); cr[7] = (R)(Eb2.im * raa);
+    R ma_[8], mb_[8];
+    frame_matrix(Fr<R>{(R)kas.ct, (R)kas.st, (R)kas.cp, (R)kas.sp}, ma_);
+    frame_matrix(Fr<R>{(R)kbs.ct, (R)kbs.st, (R)kbs.cp, (R)kbs.sp}, mb_);
+    for (int c = 0; c < 8; ++c) { cr[8 + c] = ma_[c]; cr[16 + c] = mb_[c]; }
+    // positions and path lengths scaled by K for exp2 (see cb_pass)
+    const V3<R> xb = {(R)x1d.x, (R)x1d.y, (R)x1d.z};
+    cr[24] = K * x.x; cr[25] = K * x.y; cr[26] = K * x.z;
+    cr[27] = K * xb.x; cr[28] = K * xb.y; cr[29] = K * xb.z;
+    if (GEO == 0) {
+      cr[30] = p.isinf[idx] ? R(INFINITY) : K * p.tau[idx];
+      cr[31] = p.isinf[idx1] ? R(INFINITY) : K * p.tau[idx1];
+    } else {  // τ² - x·x of the sphere's distance (sphere.jl:69-73), ≥ 0
+      cr[30] = K * K * fmax_r(p.tau[0] * p.tau[0] - dot(x, x), R(0));
+      cr[31] = K * K * fmax_r(p.tau[0] * p.tau[0] - dot(xb, xb), R(0));
+    }
+    dx[0] = xa.x - x1d.x; dx[1] = xa.y - x1d.y; dx[2] = xa.z - x1d.z;
+    dx[3] = (ta - tb) * (double)ExpK<R>::k;  // K·τab
+    cr[39] = kind_word<R>(2);
+  } else if (cb_on) {
+    // collect_cb n = 1 (cpu.jl:131-158)
+    R ma_[8];
+    frame_matrix(Fr<R>{(R)kas.ct, (R)kas.st, (R)kas.cp, (R)kas.sp}, ma_);
+    for (int c = 0; c < 8; ++c) cr[8 + c] = ma_[c];
+    const R* I = sp->I;
+    cr[0] = I[0]; cr[1] = I[1]; cr[2] = I[2]; cr[3] = I[3];
+    cr[32] = R(2) * I[1]; cr[33] = R(-2) * I[2];
+    cr[24] = K * x.x; cr[25] = K * x.y; cr[26] = K * x.z;
+    cr[30] = GEO == 0 ? (p.isinf[idx] ? R(INFINITY) : K * p.tau[idx])
+                      : K * K * fmax_r(p.tau[0] * p.tau[0] - dot(x, x), R(0));
+    cr[39] = kind_word<R>(1);
+  } else {
+    cr[39] = kind_word<R>(0);
   }
 }
 
@@ -760,31 +1032,31 @@ MCRT_DEV void stage_tables(const CoopParams<R>& p, const Lds<R>& L, int tid) {
 // a later call with resume continues them from rb = that ray; a run that ends
 // stores -2 there.  The iterations, and so the per-iteration bin sums, are
 // the same as those of an uninterrupted run.
-template <typename R, int GEO, bool MIE, bool CB, bool ATOMIC>
+template <typename R, int GEO, int SK, bool CB, bool ATOMIC>
 MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int64_t re,
                        double* slab, Slot<R>* slots, const unsigned int* preempt = nullptr,
                        bool resume = false, int64_t* state = nullptr) {
   constexpr int SLOTS = Shape<R>::kSlots;
   constexpr int SW = SLOTS / 64;  // waves owning slots
-  constexpr int CH = Shape<R>::kChunk;
-  constexpr bool STAGE_S = Shape<R>::kStageS && MIE && CB;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const bool owner = tid < p.slots;
-  R* const ldsF = L.F;
   R* const rtrec = L.rtrec;
   R* const cbrec = L.cbrec;
   double* const cbdx = L.cbdx;
   R* const esc = L.esc;
   int* const wcnt = L.wcnt;
 
-  ScaView<R> sca = p.sca;  // transport view (F/csrn in LDS)
-  if constexpr (MIE) {
+  // the transport's view of set 0 (F/csrn in LDS; kAny reads each layer's
+  // tables from HBM, lsca)
+  ScaView<R> sca = p.sca;
+  if constexpr (SK == kMie) {
     sca.F = L.F;
     sca.csrn = L.C;
   }
-  const R* Sbins = STAGE_S ? (const R*)L.S : p.sca.S;
+  const R* const Sbins = L.S ? (const R*)L.S : p.sca.S;
+  const R* const F1 = p.lo_shared ? L.F : L.F1;  // set 1's F (kAny)
   const Geo<R> geo{&p};
 
   SlotAcc* const acc = (SlotAcc*)(slots + SLOTS) + (owner ? tid : 0);
@@ -915,43 +1187,15 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
       if (owner && s.live) {
         Rng rng{p.seed, s.ray, p.instance, s.draw};
         s.n += 1;
-        const int64_t n = s.n;
-        if (n == 1) {
-          s.x1d = {(double)s.x.x, (double)s.x.y, (double)s.x.z};
-          s.idx1 = s.idx;
-        }
-        V3<double> kd = {(double)s.k.x, (double)s.k.y, (double)s.k.z};
-        if (CB) {
-          // forward_E advanced by one step (utils.jl:205-213)
-          if (n >= 2) {
-            update_E(p.sca64, s.Ea1, s.Ea2, kd, s.ksf);
-            double a = hypot(s.Ea1.re, s.Ea1.im), b = hypot(s.Ea2.re, s.Ea2.im);
-            double norm2 = sqrt(a * a + b * b);
-            double r = p.norm0 / norm2;
-            s.Ea1 = s.Ea1 * r;
-            s.Ea2 = s.Ea2 * r;
-            s.enorma += log(p.norm0 / norm2);
-            s.ksf = sph(kd);
-          }
-          // reversed path as a running product (utils.jl:230-238)
-          if (n == 2) {
-            s.Q = {{1, 0}, {0, 0}, {0, 0}, {1, 0}};
-            s.lq = 0.0;
-            s.kbs = sph(-kd);
-          } else if (n >= 3) {
-            Lmap<double> L = update_E_map(p.sca64, -s.kprev, sph(-kd));
-            s.Q = mul(s.Q, L);
-            lmap_normalise(s.Q, s.lq);
-          }
-          s.kprev = kd;
-        }
         // scatter + free path with rejection (utils.jl:294-323, cpu.jl:194-235)
+        // with the scatterer of the ray's layer (scatterer(g, idx))
         {
+          const ScaView<R> lsc = SK == kAny ? p.lsca[s.idx] : sca;
           Fr<R> ks = sph(s.k);
           for (int it = 0;; ++it) {
-            R ct = phase_sample(sca, rng.template u<R>());
-            R P[4];
-            phase_eval(sca, ct, P);
+            R ct = phase_sample(lsc, rng.template u<R>());
+            R P[6];
+            phase_eval(lsc, ct, P);
             R eq = -P[1] * s.I[1] / (P[0] * s.I[0]);
             R eu = -P[1] * s.I[2] / (P[0] * s.I[0]);
             R ee = M<R>::sqrt(eq * eq + eu * eu);
@@ -971,7 +1215,7 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
             M<R>::sincos(phi, &sph_, &cph_);
             s.kp = ray_to_norm(V3<R>{st * cph_, st * sph_, ct}, ks);
             R Ipt[4];
-            update_I(sca, s.I, s.kp, ks, Ipt);
+            update_I(lsc, s.I, s.kp, ks, Ipt);
             R rr = s.I[0] / Ipt[0];
 #pragma unroll
             for (int c = 0; c < 4; ++c) s.Ip[c] = Ipt[c] * rr;
@@ -1031,74 +1275,6 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
           esc[(e_me * kWaves + w) * 2] = R(0);
           esc[(e_me * kWaves + w) * 2 + 1] = R(0);
         }
-        if (CB) {
-          R* cr = cbrec + kCbRec * e_me;
-          const int cb_on = p.compute_cb && (near_up || near_lo);
-          Fr<double> kas = sph(kd);
-          V3<double> xa = {(double)s.x.x, (double)s.x.y, (double)s.x.z};
-          V3<double> kf0 = {p.kf0[0], p.kf0[1], p.kf0[2]};
-          if (cb_on && n > 1) {
-            // collect_cb n > 1 set-up (cpu.jl:53-81) from the O(1) state
-            double ta = dist64(p, xa, kf0, s.idx);
-            double tb = dist64(p, s.x1d, kf0, s.idx1);
-            double Id[4] = {(double)s.I[0], (double)s.I[1], (double)s.I[2], (double)s.I[3]};
-            double Iaf[4];
-            update_I(p.sca64, Id, kf0, kas, Iaf);
-            // reversed path: Q · L(-k_n | frame₀) · E_i, scaled by exp(lq + enorma)
-            Fr<double> f0 = {p.f0[0], p.f0[1], p.f0[2], p.f0[3]};
-            Lmap<double> L0 = update_E_map(p.sca64, -kd, f0);
-            Cx<double> Ei1 = {p.Ei[0], p.Ei[1]}, Ei2 = {p.Ei[2], p.Ei[3]};
-            Cx<double> u1 = L0.a * Ei1 + L0.b * Ei2, u2 = L0.c * Ei1 + L0.d * Ei2;
-            Cx<double> v1 = s.Q.a * u1 + s.Q.b * u2, v2 = s.Q.c * u1 + s.Q.d * u2;
-            double sc = exp(s.lq + s.enorma);
-            Cx<double> Eb1 = v1 * sc, Eb2 = v2 * sc;
-            Cx<double> Eaf1 = s.Ea1, Eaf2 = s.Ea2;
-            update_E(p.sca64, Eaf1, Eaf2, kf0, kas);
-            // raa (cpu.jl:68-69); zero where rounding leaves Iaf₁ ≤ 0 or the
-            // field is zero (the reference would raise DomainError or give NaN)
-            const double eaf2 = Eaf1.re * Eaf1.re + Eaf1.im * Eaf1.im + Eaf2.re * Eaf2.re +
-                                Eaf2.im * Eaf2.im;
-            const double raa = (Iaf[0] > 0 && eaf2 > 0) ? sqrt(Iaf[0] / eaf2) : 0.0;
-            cr[0] = (R)(s.Ea1.re * raa); cr[1] = (R)(s.Ea1.im * raa);
-            cr[2] = (R)(s.Ea2.re * raa); cr[3] = (R)(s.Ea2.im * raa);
-            cr[4] = (R)(Eb1.re * raa); cr[5] = (R)(Eb1.im * raa);
-            cr[6] = (R)(Eb2.re * raa); cr[7] = (R)(Eb2.im * raa);
-            R ma_[8], mb_[8];
-            frame_matrix(Fr<R>{(R)kas.ct, (R)kas.st, (R)kas.cp, (R)kas.sp}, ma_);
-            frame_matrix(Fr<R>{(R)s.kbs.ct, (R)s.kbs.st, (R)s.kbs.cp, (R)s.kbs.sp}, mb_);
-            for (int c = 0; c < 8; ++c) { cr[8 + c] = ma_[c]; cr[16 + c] = mb_[c]; }
-            // positions and path lengths scaled by K for exp2 (see cb_pass)
-            const R K = ExpK<R>::k;
-            const V3<R> xb = {(R)s.x1d.x, (R)s.x1d.y, (R)s.x1d.z};
-            cr[24] = K * s.x.x; cr[25] = K * s.x.y; cr[26] = K * s.x.z;
-            cr[27] = K * xb.x; cr[28] = K * xb.y; cr[29] = K * xb.z;
-            if (GEO == 0) {
-              cr[30] = p.isinf[s.idx] ? R(INFINITY) : K * p.tau[s.idx];
-              cr[31] = p.isinf[s.idx1] ? R(INFINITY) : K * p.tau[s.idx1];
-            } else {  // τ² - x·x of the sphere's distance (sphere.jl:69-73), ≥ 0
-              cr[30] = K * K * fmax_r(p.tau[0] * p.tau[0] - dot(s.x, s.x), R(0));
-              cr[31] = K * K * fmax_r(p.tau[0] * p.tau[0] - dot(xb, xb), R(0));
-            }
-            double* dx = cbdx + kCbDx * e_me;
-            dx[0] = xa.x - s.x1d.x; dx[1] = xa.y - s.x1d.y; dx[2] = xa.z - s.x1d.z;
-            dx[3] = (ta - tb) * (double)ExpK<R>::k;  // K·τab
-            cr[39] = kind_word<R>(2);
-          } else if (cb_on) {
-            // collect_cb n = 1 (cpu.jl:131-158)
-            R ma_[8];
-            frame_matrix(Fr<R>{(R)kas.ct, (R)kas.st, (R)kas.cp, (R)kas.sp}, ma_);
-            for (int c = 0; c < 8; ++c) cr[8 + c] = ma_[c];
-            cr[0] = s.I[0]; cr[1] = s.I[1]; cr[2] = s.I[2]; cr[3] = s.I[3];
-            cr[32] = R(2) * s.I[1]; cr[33] = R(-2) * s.I[2];
-            const R K = ExpK<R>::k;
-            cr[24] = K * s.x.x; cr[25] = K * s.x.y; cr[26] = K * s.x.z;
-            cr[30] = GEO == 0 ? (p.isinf[s.idx] ? R(INFINITY) : K * p.tau[s.idx])
-                              : K * K * fmax_r(p.tau[0] * p.tau[0] - dot(s.x, s.x), R(0));
-            cr[39] = kind_word<R>(1);
-          } else {
-            cr[39] = kind_word<R>(0);
-          }
-        }
       }
       live = s.live;
       if (owner && live) *myslot = s;
@@ -1107,17 +1283,24 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
     __syncthreads();
     const int E = wcnt[31];
 
+    // ----------------- CB path state and event records (cpu.jl:38-81), in a
+    // phase of their own so that the FP64 path state is not live across the
+    // transport's scatter loop
+    if (CB) {
+      if (owner && live)
+        cb_event<R, GEO>(p, myslot, rtrec + kRtRec * e_me, cbrec + kCbRec * e_me,
+                         cbdx + kCbDx * e_me);
+      __syncthreads();
+    }
+
     // ------------------------------- peel-off: bins × events (cpu.jl:6-31)
     if (GEO == 1 && !p.rt_cut)
-      rt_ranges<R, GEO, MIE, false>(p, ldsF, rtrec, esc, slab, E, tid, lane, wave);
+      rt_ranges<R, GEO, SK, false>(p, L.F, F1, rtrec, esc, slab, E, tid, lane, wave);
     else
-      rt_ranges<R, GEO, MIE, true>(p, ldsF, rtrec, esc, slab, E, tid, lane, wave);
+      rt_ranges<R, GEO, SK, true>(p, L.F, F1, rtrec, esc, slab, E, tid, lane, wave);
 
     // ------------------------------------------ CB bins × events (cpu.jl:38-160)
-    if (CB && p.compute_cb) {
-      if (GEO == 1 && !p.cb_cut) cb_pass<R, GEO, MIE, ATOMIC, false>(p, ldsF, Sbins, cbrec, cbdx, slab, E, tid);
-      else cb_pass<R, GEO, MIE, ATOMIC, true>(p, ldsF, Sbins, cbrec, cbdx, slab, E, tid);
-    }
+    if (CB && p.compute_cb) cb_all<R, GEO, SK, ATOMIC>(p, L.F, Sbins, cbrec, cbdx, slab, E, tid);
     __syncthreads();
     // read here, published by the barrier at the end of the iteration
     if (preempt && tid == 0)
@@ -1161,13 +1344,15 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
 
   // ---------------------------------------- energy budget and counters
   {
-    double* buf = (double*)rtrec;  // reuse the record area: 5·SLOTS doubles
-    if (tid < SLOTS)  // slots beyond p.slots were never used: they add zero
-      for (int c = 0; c < 5; ++c) buf[c * SLOTS + tid] = owner ? Aacc[c] : 0.0;
+    // reuse the record area: 5·slots doubles (it holds kRtRec·slots R words)
+    double* buf = (double*)rtrec;
+    const int ns = p.slots;
+    if (owner)
+      for (int c = 0; c < 5; ++c) buf[c * ns + tid] = Aacc[c];
     __syncthreads();
     if (tid < 5) {
       double sum = 0.0, cmp = 0.0;
-      for (int i = 0; i < SLOTS; ++i) kahan(&sum, &cmp, buf[tid * SLOTS + i]);
+      for (int i = 0; i < ns; ++i) kahan(&sum, &cmp, buf[tid * ns + i]);
       if (ATOMIC) {
         unsafeAtomicAdd(slab + tid, sum - cmp);
       } else {
@@ -1184,31 +1369,48 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
 }
 
 // One solve: workgroup g transports rays [N·g/G, N·(g+1)/G) into slab g.
-template <typename R, int GEO, bool MIE, bool CB>
+template <typename R, int GEO, int SK, bool CB>
 __global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __restrict__ pp) {
   const CoopParams<R>& p = *pp;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  const Lds<R> L = lds_layout<R, MIE, CB>(p, lds_raw);
-  stage_tables<R, MIE, CB>(p, L, threadIdx.x);
+  const Lds<R> L = lds_layout<R, SK, CB>(p, lds_raw);
+  stage_tables<R>(p, L, threadIdx.x);
   const int64_t G = gridDim.x;
   const int64_t rb = p.N * (int64_t)blockIdx.x / G;
   const int64_t re = p.N * ((int64_t)blockIdx.x + 1) / G;
   __syncthreads();
-  run_rays<R, GEO, MIE, CB, false>(p, L, rb, re, p.slabs + (size_t)blockIdx.x * 2 * p.nout,
-                            slot_block<R>(p.slot_mem, blockIdx.x));
+  run_rays<R, GEO, SK, CB, false>(p, L, rb, re, p.slabs + (size_t)blockIdx.x * 2 * p.nout,
+                                  slot_block<R>(p.slot_mem, blockIdx.x));
 }
 
-template <typename R, int GEO, bool MIE, bool CB>
+template <typename R, int GEO, int SK, bool CB>
 MCRT_DEV void batch_chunk(const CoopParams<R>& p, bool fresh, const BatchChunk& ch,
                           unsigned char* lds, Slot<R>* slots, const unsigned int* preempt,
                           int64_t* state) {
-  const Lds<R> L = lds_layout<R, MIE, CB>(p, lds);
+  const Lds<R> L = lds_layout<R, SK, CB>(p, lds);
   if (fresh) {
-    stage_tables<R, MIE, CB>(p, L, threadIdx.x);
+    stage_tables<R>(p, L, threadIdx.x);
     __syncthreads();
   }
-  run_rays<R, GEO, MIE, CB, true>(p, L, ch.rb, ch.re, ch.slab, slots, preempt, ch.resume != 0,
-                                  state);
+  run_rays<R, GEO, SK, CB, true>(p, L, ch.rb, ch.re, ch.slab, slots, preempt, ch.resume != 0,
+                                 state);
+}
+
+template <typename R, int GEO, int SK>
+MCRT_DEV void batch_chunk_cb(const CoopParams<R>& p, bool fresh, const BatchChunk& ch,
+                             unsigned char* lds, Slot<R>* slots, const unsigned int* preempt,
+                             int64_t* state) {
+  if (p.compute_cb) batch_chunk<R, GEO, SK, true>(p, fresh, ch, lds, slots, preempt, state);
+  else batch_chunk<R, GEO, SK, false>(p, fresh, ch, lds, slots, preempt, state);
+}
+
+template <typename R, int GEO>
+MCRT_DEV void batch_chunk_geo(const CoopParams<R>& p, bool fresh, const BatchChunk& ch,
+                              unsigned char* lds, Slot<R>* slots, const unsigned int* preempt,
+                              int64_t* state) {
+  if (p.sk == kMie) batch_chunk_cb<R, GEO, kMie>(p, fresh, ch, lds, slots, preempt, state);
+  else if (p.sk == kRay) batch_chunk_cb<R, GEO, kRay>(p, fresh, ch, lds, slots, preempt, state);
+  else batch_chunk_cb<R, GEO, kAny>(p, fresh, ch, lds, slots, preempt, state);
 }
 
 // Many solves in one persistent launch: each workgroup takes the next chunk
@@ -1244,36 +1446,24 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(const BatchChunk* __res
     cur = ch.params;
     Slot<R>* const slots = ch.cont ? (Slot<R>*)ch.cont : own;
     int64_t* const st = states ? states + c : nullptr;
-    const bool mie = p.sca.kind != 0, cb = p.compute_cb != 0;
-    if (p.geo_kind == 0) {
-      if (mie) {
-        if (cb) batch_chunk<R, 0, true, true>(p, fresh, ch, lds_raw, slots, preempt, st);
-        else batch_chunk<R, 0, true, false>(p, fresh, ch, lds_raw, slots, preempt, st);
-      } else {
-        if (cb) batch_chunk<R, 0, false, true>(p, fresh, ch, lds_raw, slots, preempt, st);
-        else batch_chunk<R, 0, false, false>(p, fresh, ch, lds_raw, slots, preempt, st);
-      }
-    } else {
-      if (mie) {
-        if (cb) batch_chunk<R, 1, true, true>(p, fresh, ch, lds_raw, slots, preempt, st);
-        else batch_chunk<R, 1, true, false>(p, fresh, ch, lds_raw, slots, preempt, st);
-      } else {
-        if (cb) batch_chunk<R, 1, false, true>(p, fresh, ch, lds_raw, slots, preempt, st);
-        else batch_chunk<R, 1, false, false>(p, fresh, ch, lds_raw, slots, preempt, st);
-      }
-    }
+    if (p.geo_kind == 0) batch_chunk_geo<R, 0>(p, fresh, ch, lds_raw, slots, preempt, st);
+    else batch_chunk_geo<R, 1>(p, fresh, ch, lds_raw, slots, preempt, st);
   }
 }
 
-// Known-answer evaluation (mcrt_plan_eval): the building blocks above on n
-// fixed records, converted to R on entry.  UPDATE_I/UPDATE_E records carry the
-// bin constants of k in words 11..15 (Eh₁, Eh₂, Eh × k), as the bin tables do.
-template <typename R, bool MIE>
+// Known-answer evaluation (mcrt_plan_eval): the building blocks above with
+// table set 0 (kind SK) on n fixed records, converted to R on entry.
+// UPDATE_I/UPDATE_E records carry the bin constants of k in words 11..15
+// (Eh₁, Eh₂, Eh × k), as the bin tables do.
+template <typename R, int SK>
 __global__ void eval_kernel(const CoopParams<R>* __restrict__ pp, int what, int64_t n,
                             const double* __restrict__ in, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const CoopParams<R>& p = *pp;
+  const TabC<R>& t = p.tc[0];
+  // the transport's view of layer 1's scatterer
+  const ScaView<R> tsc = p.sk == kAny ? p.lsca[0] : p.sca;
   const double* a = in + 16 * i;
   double* o = out + 4 * i;
   R r[4] = {R(0), R(0), R(0), R(0)};
@@ -1289,12 +1479,12 @@ __global__ void eval_kernel(const CoopParams<R>* __restrict__ pp, int what, int6
       if (what == MCRT_EVAL_UPDATE_I) {
         const R I[4] = {(R)a[4], (R)a[5], (R)a[6], (R)a[7]};
         const R J[2] = {R(2) * I[1], R(-2) * I[2]};
-        peel_I<R, MIE>(p, p.sca.F, I, J, q, r);
+        peel_I<R, SK>(t, p.sca.F, I, J, q, r);
         r[1] = r[1] * q.inv;
         r[2] = r[2] * q.inv;
       } else {
         Cx<R> E1 = {(R)a[4], (R)a[5]}, E2 = {(R)a[6], (R)a[7]};
-        peel_E<R, MIE>(p, p.sca.S, q, q.inv, E1, E2);
+        peel_E<R, SK, true>(t, p.sca.S, q, q.inv, E1, E2);
         r[0] = E1.re; r[1] = E1.im; r[2] = E2.re; r[3] = E2.im;
       }
       break;
@@ -1314,21 +1504,33 @@ __global__ void eval_kernel(const CoopParams<R>* __restrict__ pp, int what, int6
       dbl = true;
       break;
     }
-    case MCRT_EVAL_PHASE:
-      phase_tab<R, MIE>(p, p.sca.F, (R)a[0], r);
+    case MCRT_EVAL_PHASE: {
+      const Ph<R> P = phase_tab<R, SK>(t, p.sca.F, (R)a[0]);
+      r[0] = P.p11; r[1] = P.p12; r[2] = P.p33; r[3] = P.p34;
       break;
+    }
     case MCRT_EVAL_AMPLITUDE: {
       Cx<R> S11, S22;
-      amp_tab<R, MIE>(p, p.sca.S, (R)a[0], S11, S22);
+      amp_tab<R, SK>(t, p.sca.S, (R)a[0], S11, S22);
       r[0] = S11.re; r[1] = S11.im; r[2] = S22.re; r[3] = S22.im;
       break;
     }
     case MCRT_EVAL_SAMPLE:
-      r[0] = phase_sample(p.sca, (R)a[0]);
+      r[0] = phase_sample(tsc, (R)a[0]);
       break;
-    case MCRT_EVAL_PHASE_TRANSPORT:
-      phase_eval(p.sca, (R)a[0], r);
+    case MCRT_EVAL_PHASE_TRANSPORT: {
+      R P[6];
+      phase_eval(tsc, (R)a[0], P);
+      r[0] = P[0]; r[1] = P[1]; r[2] = P[3]; r[3] = P[4];
       break;
+    }
+    case MCRT_EVAL_PHASE_DIAG: {
+      const Ph<R> P = phase_tab<R, SK>(t, p.sca.F, (R)a[0]);
+      R Q[6];
+      phase_eval(tsc, (R)a[0], Q);
+      r[0] = P.p22; r[1] = P.p44; r[2] = Q[2]; r[3] = Q[5];
+      break;
+    }
     case MCRT_EVAL_KEPLER:
       r[0] = kepler((R)a[0], (R)a[1]);
       break;
@@ -1338,6 +1540,7 @@ __global__ void eval_kernel(const CoopParams<R>* __restrict__ pp, int what, int6
   for (int c = 0; c < 4; ++c) o[c] = dbl ? d[c] : (double)r[c];
 }
 
+#if MCRT_PART(1)
 // The value of output i in one private slab: output words A and Irt map
 // one-to-one, a CB word is the sum over the slab's CB groups.
 MCRT_DEV double slab_value(const double* sl, int64_t sn, int64_t i, int64_t nrt, int64_t ncb,
@@ -1476,7 +1679,23 @@ void h_E_from_I(const double I[4], double E[4]) {
   E[0] = A1; E[1] = A1 * 0.0; E[2] = A2 * std::cos(ph); E[3] = A2 * std::sin(ph);
 }
 
+#endif  // MCRT_PART(1)
+
 }  // namespace
+
+// Launchers of the templated kernels, each instantiated in one part (defined
+// at the end of this file).
+template <typename R, int GEO> hipError_t coop_launch_geo(const CoopPlan& cp, hipStream_t s);
+template <typename R> hipError_t batch_set_lds_r(size_t lds_bytes);
+template <typename R>
+hipError_t batch_launch_r(const BatchChunk* dchunks, int nchunks, unsigned int* dnext,
+                          void* slot_mem, int blocks, size_t lds_bytes, hipStream_t s,
+                          int64_t* states, const unsigned int* preempt);
+template <typename R>
+hipError_t eval_launch_r(const CoopPlan& cp, int what, int64_t n, const double* din, double* dout,
+                         hipStream_t s);
+
+#if MCRT_PART(1)
 
 // ------------------------------------------------------------- table cache
 struct TableCache {
@@ -1551,11 +1770,13 @@ uint64_t table_hash(const mcrt_scatterer* s) {
     const unsigned char* b = (const unsigned char*)p;
     for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
   };
+  mix(&s->kind, sizeof s->kind);
   mix(&s->n_theta, sizeof s->n_theta);
   mix(&s->random_points, sizeof s->random_points);
   const size_t N = (size_t)s->n_theta;
   mix(s->F11, 8 * N); mix(s->F12, 8 * N); mix(s->F33, 8 * N); mix(s->F34, 8 * N);
   mix(s->S11, 16 * N); mix(s->S22, 16 * N);
+  if (s->kind == MCRT_SCATTERER_SPLINE) { mix(s->F22, 8 * N); mix(s->F44, 8 * N); }
   mix(s->csrn, 8 * ((size_t)s->random_points + 1));
   return h;
 }
@@ -1566,10 +1787,10 @@ uint64_t table_hash(const mcrt_scatterer* s) {
 template <typename R>
 void fill_sca(ScaView<R>& v, const mcrt_scatterer* s, const DevAlloc& alloc, int* rc, std::string* err,
               TableCache* cache = nullptr, std::vector<const void*>* refs = nullptr) {
-  v.kind = s->kind == MCRT_SCATTERER_MIE ? 1 : 0;
+  v.kind = s->kind == MCRT_SCATTERER_MIE ? kMie : (s->kind == MCRT_SCATTERER_SPLINE ? kSpl : kRay);
   v.Nt = s->n_theta;
   v.rp = s->random_points;
-  v.scale = 0;
+  v.scale = v.kind ? (R)((s->n_theta - 1) / 3.141592653589793) : R(0);
   v.csrn = v.F = v.S = nullptr;
   if (!v.kind) return;
   std::unique_lock<std::mutex> lk;
@@ -1592,11 +1813,17 @@ void fill_sca(ScaView<R>& v, const mcrt_scatterer* s, const DevAlloc& alloc, int
     ++cache->misses;
   }
   const int N = s->n_theta;
-  std::vector<R> F(4 * (size_t)(N + 1), R(0)), S(4 * (size_t)(N + 1), R(0)),
+  const int fs = f_stride(v.kind);  // (F11, F12, F33, F34) or (F11, F12, F22, F33, F34, F44, 0, 0)
+  std::vector<R> F((size_t)fs * (N + 1), R(0)), S(4 * (size_t)(N + 1), R(0)),
       C((size_t)s->random_points + 1);
   for (int i = 0; i < N; ++i) {
-    F[4 * i] = (R)s->F11[i]; F[4 * i + 1] = (R)s->F12[i];
-    F[4 * i + 2] = (R)s->F33[i]; F[4 * i + 3] = (R)s->F34[i];
+    R* f = &F[(size_t)fs * i];
+    if (fs == 4) {
+      f[0] = (R)s->F11[i]; f[1] = (R)s->F12[i]; f[2] = (R)s->F33[i]; f[3] = (R)s->F34[i];
+    } else {
+      f[0] = (R)s->F11[i]; f[1] = (R)s->F12[i]; f[2] = (R)s->F22[i];
+      f[3] = (R)s->F33[i]; f[4] = (R)s->F34[i]; f[5] = (R)s->F44[i];
+    }
     S[4 * i] = (R)s->S11[2 * i]; S[4 * i + 1] = (R)s->S11[2 * i + 1];
     S[4 * i + 2] = (R)s->S22[2 * i]; S[4 * i + 3] = (R)s->S22[2 * i + 1];
   }
@@ -1640,41 +1867,102 @@ void fill_sca(ScaView<R>& v, const mcrt_scatterer* s, const DevAlloc& alloc, int
 }
 
 // LDS bytes of one workgroup (the layout of coop_kernel).
-template <typename R> size_t lds_plan(bool mie, bool cb, int nt, int rp) {
-  constexpr int SLOTS = Shape<R>::kSlots;
-  auto al = [](size_t b) { return (b + 15) & ~size_t(15); };
-  size_t b = al(sizeof(R) * kRtRec * SLOTS);
-  if (cb) b += al(sizeof(R) * kCbRec * SLOTS) + al(sizeof(double) * kCbDx * SLOTS);
-  b += al(sizeof(R) * SLOTS * kWaves * 2) + al(sizeof(int) * 32);
-  if (mie) b += al(sizeof(R) * 4 * (nt + 1)) + al(sizeof(R) * (rp + 1));
-  if (mie && cb && Shape<R>::kStageS) b += al(sizeof(R) * 4 * (nt + 1));
-  return b;
+template <typename R> size_t lds_need(const CoopParams<R>& P) {
+  const bool cb = P.compute_cb != 0;
+  switch (P.sk) {
+    case kRay: return cb ? lds_offsets<R, kRay, true>(P).end : lds_offsets<R, kRay, false>(P).end;
+    case kMie: return cb ? lds_offsets<R, kMie, true>(P).end : lds_offsets<R, kMie, false>(P).end;
+    default: return cb ? lds_offsets<R, kAny, true>(P).end : lds_offsets<R, kAny, false>(P).end;
+  }
+}
+
+// The LDS shape of a solve whose table fields are set: the most slots (a
+// multiple of 64, at most Shape<R>::kSlots and at most max_slots when > 0) whose
+// records fit beside the tables in lds_max bytes, with the S table staged when
+// want_s and it fits at that many slots (first choice), else read from HBM.
+// Returns false when even 64 slots with S in HBM do not fit.
+template <typename R> bool fit_lds(CoopParams<R>& P, size_t lds_max, int max_slots, bool want_s,
+                                   size_t* lds_out) {
+  const int top = max_slots > 0 ? std::min(Shape<R>::kSlots, std::max(64, max_slots / 64 * 64))
+                                : Shape<R>::kSlots;
+  const bool can_s = Shape<R>::kStageS && P.compute_cb && P.sca.kind != kRay && want_s;
+  for (int stage = can_s ? 1 : 0; stage >= 0; --stage)
+    for (int slots = top; slots >= 64; slots -= 64) {
+      P.slots = slots;
+      P.stage_s = stage;
+      const size_t b = lds_need(P);
+      // keep the staged S only if it leaves the full slot count
+      if (b <= lds_max && (stage == 0 || slots == top)) {
+        *lds_out = b;
+        return true;
+      }
+    }
+  return false;
+}
+
+// Kernel kind of a solve: kRay / kMie when one Rayleigh or Mie scatterer fills
+// every layer, else kAny (layers with different scatterers, or a Spline).
+inline int solve_kind(const mcrt_geometry* g) {
+  const mcrt_scatterer* s0 = g->kind == MCRT_GEOMETRY_PLANE ? g->layers[0].scatterer : g->scatterer;
+  const int nl = g->kind == MCRT_GEOMETRY_PLANE ? g->n_layers : 1;
+  bool same = true, spline = s0->kind == MCRT_SCATTERER_SPLINE;
+  for (int i = 1; i < nl; ++i) {
+    same = same && g->layers[i].scatterer == s0;
+    spline = spline || g->layers[i].scatterer->kind == MCRT_SCATTERER_SPLINE;
+  }
+  if (spline || !same) return kAny;
+  return s0->kind == MCRT_SCATTERER_MIE ? kMie : kRay;
+}
+
+// Lookup constants of a table of n_theta nodes.
+template <typename R> TabC<R> tab_consts(const ScaView<R>& v) {
+  TabC<R> t{};
+  if (v.kind == kRay) return t;
+  const double sc = (v.Nt - 1) / 3.141592653589793;
+  // acos(a) = √(1-a)·Σ cₖ aᵏ on [0, 1]: the degree-6 minimax fit printed by
+  // csrc/fit_acos.py, highest degree first
+  static const double c[kAcosTerms] = {0.0026117211109719563, -0.012003396496319869,
+                                       0.027762914927838391,  -0.049197437593977882,
+                                       0.088835885904433951,  -0.21459108867891987,
+                                       1.5707962388530227};
+  t.scale = (R)sc;
+  for (int k = 0; k < kAcosTerms; ++k) t.acs[k] = (R)(c[k] * sc);
+  t.pis = (R)(v.Nt - 1);  // π·sc
+  return t;
 }
 
 // Fill the parameter block of one solve and upload its tables and bin
 // constants (everything but the launch resources: slabs and slot state).
+// P.slots is the most slots the solve's LDS allows (at most max_slots when
+// > 0), *lds_out the LDS bytes at that count.
 template <typename R>
 int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_config* cfg,
                 const mcrt_geometry* g, const double I0[4], uint64_t seed, uint64_t ray0,
                 uint32_t instance, const DevAlloc& alloc, unsigned long long* dcnt,
                 std::string* err, TableCache* cache = nullptr,
-                std::vector<const void*>* refs = nullptr) {
+                std::vector<const void*>* refs = nullptr, int max_slots = 0, bool want_s = true) {
   std::memset(&P, 0, sizeof P);
-  // one scatterer for all layers
-  const mcrt_scatterer* s0 = g->kind == MCRT_GEOMETRY_PLANE ? g->layers[0].scatterer : g->scatterer;
-  if (g->kind == MCRT_GEOMETRY_PLANE)
-    for (int i = 1; i < g->n_layers; ++i)
-      if (g->layers[i].scatterer != s0) {
-        *err = "the cooperative forms need one scatterer shared by all layers (use fp64_plain)";
-        return MCRT_ERR_UNSUPPORTED;
-      }
-  const bool mie = s0->kind == MCRT_SCATTERER_MIE;
+  const bool plane = g->kind == MCRT_GEOMETRY_PLANE;
+  const mcrt_scatterer* s0 = plane ? g->layers[0].scatterer : g->scatterer;
+  const mcrt_scatterer* s1 = plane ? g->layers[g->n_layers - 1].scatterer : s0;
   const bool cb = cfg->compute_cb != 0;
-  const size_t lds = lds_plan<R>(mie, cb, mie ? s0->n_theta : 0, mie ? s0->random_points : 0);
-  if (lds > lds_max) {
-    *err = "the Mie tables do not fit in LDS beside the event records (reduce Nθ or random_points, "
-           "or use fp64_plain)";
-    return MCRT_ERR_UNSUPPORTED;
+  P.sk = solve_kind(g);
+  P.compute_cb = cb ? 1 : 0;
+  P.lo_shared = s1 == s0;
+  {
+    // the table fields the LDS layout reads, before anything is uploaded
+    auto shape_of = [](ScaView<R>& v, const mcrt_scatterer* s) {
+      v.kind = s->kind == MCRT_SCATTERER_MIE ? kMie : (s->kind == MCRT_SCATTERER_SPLINE ? kSpl : kRay);
+      v.Nt = s->n_theta;
+      v.rp = s->random_points;
+    };
+    shape_of(P.sca, s0);
+    shape_of(P.sca_lo, s1);
+    if (!fit_lds(P, lds_max, max_slots, want_s, lds_out)) {
+      *err = "the scatterer tables do not fit in LDS beside the event records of 64 ray slots "
+             "(reduce Nθ or random_points, or use fp64_plain)";
+      return MCRT_ERR_UNSUPPORTED;
+    }
   }
   int layer_slot[kMaxLayers] = {};
   GeoDev gd = make_geo(g, layer_slot);
@@ -1691,21 +1979,29 @@ int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_co
   }
   P.kl1 = gd.kl1;
   int rc = MCRT_OK;
-  fill_sca<R>(P.sca, s0, alloc, &rc, err, cache, refs);
-  if (rc) return rc;
+  {
+    // tables: set 0, set 1 and (kAny) every layer's, each distinct scatterer once
+    std::vector<std::pair<const mcrt_scatterer*, ScaView<R>>> done;
+    auto view = [&](const mcrt_scatterer* sc, ScaView<R>& v) {
+      for (auto& d : done)
+        if (d.first == sc) { v = d.second; return; }
+      fill_sca<R>(v, sc, alloc, &rc, err, cache, refs);
+      if (!rc) done.push_back({sc, v});
+    };
+    const int slots = P.slots, stage_s = P.stage_s;
+    view(s0, P.sca);
+    if (!rc) view(s1, P.sca_lo);
+    if (!rc && P.sk == kAny)
+      for (int i = 0; i < gd.nl && !rc; ++i) view(plane ? g->layers[i].scatterer : s0, P.lsca[i]);
+    if (rc) return rc;
+    P.slots = slots;
+    P.stage_s = stage_s;
+  }
   fill_sca<double>(P.sca64, s0, alloc, &rc, err, cache, refs);
   if (rc) return rc;
   {
-    const double sc = mie ? (s0->n_theta - 1) / 3.141592653589793 : 0.0;
-    // acos(a) = √(1-a)·Σ cₖ aᵏ on [0, 1]: the degree-6 minimax fit printed by
-    // csrc/fit_acos.py, highest degree first
-    static const double c[kAcosTerms] = {0.0026117211109719563, -0.012003396496319869,
-                                         0.027762914927838391,  -0.049197437593977882,
-                                         0.088835885904433951,  -0.21459108867891987,
-                                         1.5707962388530227};
-    P.scale = (R)sc;
-    for (int k = 0; k < kAcosTerms; ++k) P.acs[k] = (R)(c[k] * sc);
-    P.pis = mie ? (R)(s0->n_theta - 1) : R(0);  // π·sc
+    P.tc[0] = tab_consts(P.sca);
+    P.tc[1] = tab_consts(P.sca_lo);
     P.tau0k = (R)(cfg->tau0 * (double)ExpK<R>::k);
     // inside a sphere every distance to the boundary is at most the diameter
     P.rt_cut = !(g->kind == MCRT_GEOMETRY_SPHERE && 2.0 * g->tau * (1.0 + 1e-6) < cfg->tau0);
@@ -1810,7 +2106,6 @@ int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_co
     P.cb_groups = best;
   }
   P.counters = dcnt;
-  *lds_out = lds;
   return MCRT_OK;
 }
 
@@ -1822,9 +2117,8 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
   CoopParams<R> P;
   size_t lds = 0;
   int rc = init_params<R>(P, &lds, lds_max, cfg, g, I0, seed, ray0, instance, alloc, dcnt, err,
-                          cache, &cp.table_refs);
+                          cache, &cp.table_refs, shape.slots, shape.stage_s >= 0);
   if (rc) return rc;
-  const bool mie = P.sca.kind != 0;
   const bool cb = P.compute_cb != 0;
   auto up = [&](const void* h, size_t bytes) -> void* {
     hipError_t e;
@@ -1835,23 +2129,24 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
     }
     return d;
   };
-  // launch shape: workgroups per CU (LDS bounds how many are resident) and
-  // active slots per workgroup; by default full slots, fewer when the solve
-  // has too few rays to give every CU a workgroup
+  // launch shape: active slots per workgroup (at most what LDS allows, P.slots
+  // from init_params; by default all of them, fewer when the solve has too few
+  // rays to give every CU a workgroup), the LDS it needs, and workgroups per
+  // CU (LDS bounds how many are resident)
+  int slots = P.slots;
+  if (shape.slots <= 0 && (P.N + slots - 1) / slots < cu_count) {
+    const int64_t per = (P.N + cu_count - 1) / cu_count;
+    slots = (int)std::min<int64_t>(slots, std::max<int64_t>(64, (per + 63) / 64 * 64));
+  }
+  P.slots = slots;
+  lds = lds_need(P);
   cp.lds_bytes = lds;
   cp.lds_max = lds_max;
   const int wg_per_cu = std::max<int>(1, (int)(lds_max / std::max<size_t>(lds, 1)));
   const int per_cu = shape.workgroups_per_cu > 0 ? std::min(shape.workgroups_per_cu, 4)
                                                  : std::min(wg_per_cu, 2);
-  int slots = Shape<R>::kSlots;
-  if (shape.slots > 0) {
-    slots = std::min<int>(Shape<R>::kSlots, std::max(64, (shape.slots + 63) / 64 * 64));
-  } else if ((P.N + slots - 1) / slots < cu_count) {
-    const int64_t per = (P.N + cu_count - 1) / cu_count;
-    slots = (int)std::min<int64_t>(Shape<R>::kSlots, std::max<int64_t>(64, (per + 63) / 64 * 64));
-  }
-  P.slots = slots;
   cp.slots = slots;
+  cp.stage_s = P.stage_s;
   cp.wg_per_cu = per_cu;
   const int64_t want = (int64_t)cu_count * per_cu;
   const int64_t byrays = (P.N + slots - 1) / slots;
@@ -1875,32 +2170,10 @@ int create_impl(CoopPlan& cp, int cu_count, size_t lds_max, const mcrt_config* c
   cp.params_dev = up(&P, sizeof P);
   if (rc) return rc;
   cp.geo_kind = P.geo_kind;
-  cp.mie = mie;
+  cp.sk = P.sk;
+  cp.sk0 = P.sca.kind;
   cp.cb = cb;
   return MCRT_OK;
-}
-
-template <typename R, int GEO, bool MIE, bool CB>
-hipError_t launch_t(const CoopPlan& cp, hipStream_t s) {
-  auto k = coop_kernel<R, GEO, MIE, CB>;
-  // the attribute is per kernel and process-wide: the device maximum, so that
-  // plans of other contexts and threads never see a smaller value set here
-  hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)cp.lds_max);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(cp.blocks), dim3(cp.threads), cp.lds_bytes, s,
-                     (const CoopParams<R>*)cp.params_dev);
-  return hipGetLastError();
-}
-
-template <typename R>
-hipError_t launch_r(const CoopPlan& cp, hipStream_t s) {
-  if (cp.geo_kind == 0) {
-    if (cp.mie) return cp.cb ? launch_t<R, 0, true, true>(cp, s) : launch_t<R, 0, true, false>(cp, s);
-    return cp.cb ? launch_t<R, 0, false, true>(cp, s) : launch_t<R, 0, false, false>(cp, s);
-  }
-  if (cp.mie) return cp.cb ? launch_t<R, 1, true, true>(cp, s) : launch_t<R, 1, true, false>(cp, s);
-  return cp.cb ? launch_t<R, 1, false, true>(cp, s) : launch_t<R, 1, false, false>(cp, s);
 }
 
 }  // namespace
@@ -1922,7 +2195,7 @@ int coop_create(CoopPlan& cp, hipStream_t, int cu_count, size_t lds_max, const m
 
 int table_pin(TableCache* cache, const mcrt_scatterer* s, std::vector<const void*>* refs,
               std::string* err) {
-  if (s->kind != MCRT_SCATTERER_MIE) return MCRT_OK;
+  if (s->kind == MCRT_SCATTERER_RAYLEIGH) return MCRT_OK;
   const DevAlloc none = [](size_t, hipError_t* e) -> void* { *e = hipErrorInvalidValue; return nullptr; };
   int rc = MCRT_OK;
   ScaView<float> vf;
@@ -1938,7 +2211,8 @@ hipError_t coop_prepare(CoopPlan& cp, hipStream_t s) {
 }
 
 hipError_t coop_launch(CoopPlan& cp, hipStream_t s) {
-  return cp.mixed ? launch_r<float>(cp, s) : launch_r<double>(cp, s);
+  if (cp.mixed) return cp.geo_kind == 0 ? coop_launch_geo<float, 0>(cp, s) : coop_launch_geo<float, 1>(cp, s);
+  return cp.geo_kind == 0 ? coop_launch_geo<double, 0>(cp, s) : coop_launch_geo<double, 1>(cp, s);
 }
 
 // ------------------------------------------------------------- batched solves
@@ -2013,10 +2287,9 @@ int solve_create_impl(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, cons
   int rc = init_params<R>(P, &lds, lds_max, cfg, g, I0, seed, ray0, instance, alloc, sd.counters, err,
                           cache, &sd.table_refs);
   if (rc) return rc;
-  P.slots = Shape<R>::kSlots;
+  // P.slots: the most slots the solve's LDS allows
   sd.mixed = sizeof(R) == 4;
   sd.geo_kind = P.geo_kind;
-  sd.mie = P.sca.kind != 0;
   sd.cb = P.compute_cb != 0;
   sd.N = P.N;
   sd.nout = P.nout;
@@ -2114,8 +2387,7 @@ size_t batch_slot_bytes(bool mixed, int blocks) {
 }
 
 hipError_t batch_set_lds(bool mixed, size_t lds_bytes) {
-  const void* k = mixed ? (const void*)batch_kernel<float> : (const void*)batch_kernel<double>;
-  return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  return mixed ? batch_set_lds_r<float>(lds_bytes) : batch_set_lds_r<double>(lds_bytes);
 }
 
 hipError_t batch_launch(bool mixed, const BatchChunk* dchunks, int nchunks, unsigned int* dnext,
@@ -2125,13 +2397,10 @@ hipError_t batch_launch(bool mixed, const BatchChunk* dchunks, int nchunks, unsi
     hipError_t e = zero(dnext, sizeof(unsigned int), s);
     if (e != hipSuccess) return e;
   }
-  if (mixed)
-    hipLaunchKernelGGL(batch_kernel<float>, dim3(blocks), dim3(kThreads), lds_bytes, s, dchunks,
-                       nchunks, dnext, slot_mem, states, preempt);
-  else
-    hipLaunchKernelGGL(batch_kernel<double>, dim3(blocks), dim3(kThreads), lds_bytes, s, dchunks,
-                       nchunks, dnext, slot_mem, states, preempt);
-  return hipGetLastError();
+  return mixed ? batch_launch_r<float>(dchunks, nchunks, dnext, slot_mem, blocks, lds_bytes, s,
+                                       states, preempt)
+               : batch_launch_r<double>(dchunks, nchunks, dnext, slot_mem, blocks, lds_bytes, s,
+                                        states, preempt);
 }
 
 hipError_t coop_sigma(CoopPlan& cp, hipStream_t s, int64_t N, double* sig) {
@@ -2160,22 +2429,9 @@ int coop_eval(const CoopPlan& cp, hipStream_t s, int what, int64_t n, const doub
   hipError_t e = hipMalloc(&din, sizeof(double) * h.size());
   if (e == hipSuccess) e = hipMalloc(&dout, sizeof(double) * 4 * (size_t)n);
   if (e == hipSuccess) e = hipMemcpyAsync(din, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) {
-    const int th = 256;
-    const int bl = (int)((n + th - 1) / th);
-    const double* di = (const double*)din;
-    double* dd = (double*)dout;
-    if (cp.mixed) {
-      const auto* P = (const CoopParams<float>*)cp.params_dev;
-      if (cp.mie) hipLaunchKernelGGL((eval_kernel<float, true>), dim3(bl), dim3(th), 0, s, P, what, n, di, dd);
-      else hipLaunchKernelGGL((eval_kernel<float, false>), dim3(bl), dim3(th), 0, s, P, what, n, di, dd);
-    } else {
-      const auto* P = (const CoopParams<double>*)cp.params_dev;
-      if (cp.mie) hipLaunchKernelGGL((eval_kernel<double, true>), dim3(bl), dim3(th), 0, s, P, what, n, di, dd);
-      else hipLaunchKernelGGL((eval_kernel<double, false>), dim3(bl), dim3(th), 0, s, P, what, n, di, dd);
-    }
-    e = hipGetLastError();
-  }
+  if (e == hipSuccess)
+    e = cp.mixed ? eval_launch_r<float>(cp, what, n, (const double*)din, (double*)dout, s)
+                 : eval_launch_r<double>(cp, what, n, (const double*)din, (double*)dout, s);
   if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   (void)hipFree(din);
@@ -2194,5 +2450,91 @@ hipError_t coop_reduce(CoopPlan& cp, hipStream_t s) {
                      cp.slab_words, cp.nout, cp.dA, cp.dIrt, cp.dIcb, cp.nrt, cp.ncb, cp.cb_groups);
   return hipGetLastError();
 }
+
+#endif  // MCRT_PART(1)
+
+// ------------------------------------------------------- kernel launchers
+#if MCRT_COOP_PART != 1
+namespace {
+template <typename R, int GEO, int SK, bool CB>
+hipError_t launch_t(const CoopPlan& cp, hipStream_t s) {
+  auto k = coop_kernel<R, GEO, SK, CB>;
+  // the attribute is per kernel and process-wide: the device maximum, so that
+  // plans of other contexts and threads never see a smaller value set here
+  hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)cp.lds_max);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(cp.blocks), dim3(cp.threads), cp.lds_bytes, s,
+                     (const CoopParams<R>*)cp.params_dev);
+  return hipGetLastError();
+}
+
+template <typename R, int GEO, int SK>
+hipError_t launch_cb(const CoopPlan& cp, hipStream_t s) {
+  return cp.cb ? launch_t<R, GEO, SK, true>(cp, s) : launch_t<R, GEO, SK, false>(cp, s);
+}
+template <typename R, int GEO>
+hipError_t launch_geo(const CoopPlan& cp, hipStream_t s) {
+  if (cp.sk == kMie) return launch_cb<R, GEO, kMie>(cp, s);
+  if (cp.sk == kRay) return launch_cb<R, GEO, kRay>(cp, s);
+  return launch_cb<R, GEO, kAny>(cp, s);
+}
+
+}  // namespace
+
+template <typename R, int GEO> hipError_t coop_launch_geo(const CoopPlan& cp, hipStream_t s) {
+  return launch_geo<R, GEO>(cp, s);
+}
+
+template <typename R> hipError_t batch_set_lds_r(size_t lds_bytes) {
+  return hipFuncSetAttribute((const void*)batch_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds_bytes);
+}
+
+template <typename R>
+hipError_t batch_launch_r(const BatchChunk* dchunks, int nchunks, unsigned int* dnext,
+                          void* slot_mem, int blocks, size_t lds_bytes, hipStream_t s,
+                          int64_t* states, const unsigned int* preempt) {
+  hipLaunchKernelGGL(batch_kernel<R>, dim3(blocks), dim3(kThreads), lds_bytes, s, dchunks, nchunks,
+                     dnext, slot_mem, states, preempt);
+  return hipGetLastError();
+}
+
+// The building blocks of table set 0 (layer 1's scatterer).
+template <typename R>
+hipError_t eval_launch_r(const CoopPlan& cp, int what, int64_t n, const double* din, double* dout,
+                         hipStream_t s) {
+  const int th = 256;
+  const int bl = (int)((n + th - 1) / th);
+  const auto* P = (const CoopParams<R>*)cp.params_dev;
+  if (cp.sk0 == kMie) hipLaunchKernelGGL((eval_kernel<R, kMie>), dim3(bl), dim3(th), 0, s, P, what, n, din, dout);
+  else if (cp.sk0 == kSpl) hipLaunchKernelGGL((eval_kernel<R, kSpl>), dim3(bl), dim3(th), 0, s, P, what, n, din, dout);
+  else hipLaunchKernelGGL((eval_kernel<R, kRay>), dim3(bl), dim3(th), 0, s, P, what, n, din, dout);
+  return hipGetLastError();
+}
+#endif
+
+#if MCRT_COOP_PART == 2
+template hipError_t coop_launch_geo<float, 0>(const CoopPlan&, hipStream_t);
+#endif
+#if MCRT_COOP_PART == 3
+template hipError_t coop_launch_geo<float, 1>(const CoopPlan&, hipStream_t);
+#endif
+#if MCRT_COOP_PART == 4
+template hipError_t batch_set_lds_r<float>(size_t);
+template hipError_t batch_launch_r<float>(const BatchChunk*, int, unsigned int*, void*, int, size_t,
+                                          hipStream_t, int64_t*, const unsigned int*);
+template hipError_t eval_launch_r<float>(const CoopPlan&, int, int64_t, const double*, double*,
+                                         hipStream_t);
+#endif
+#if MCRT_COOP_PART == 5
+template hipError_t coop_launch_geo<double, 0>(const CoopPlan&, hipStream_t);
+template hipError_t coop_launch_geo<double, 1>(const CoopPlan&, hipStream_t);
+template hipError_t batch_set_lds_r<double>(size_t);
+template hipError_t batch_launch_r<double>(const BatchChunk*, int, unsigned int*, void*, int, size_t,
+                                           hipStream_t, int64_t*, const unsigned int*);
+template hipError_t eval_launch_r<double>(const CoopPlan&, int, int64_t, const double*, double*,
+                                          hipStream_t);
+#endif
 
 }  // namespace mcrt

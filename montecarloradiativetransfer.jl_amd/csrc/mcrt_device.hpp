@@ -7,7 +7,8 @@
 //   get_basis 52-62, rotate 82-84, update_I 91-112, E_from_I 119-150,
 //   I_from_E 157-163, update_E 170-192, kepler_solver 243-261,
 //   scatter 294-323; Rayleigh phase/amplitude (scatterers/rayleigh.jl:8-23);
-//   Mie phase/amplitude lookup (scatterers/mie.jl:46-92).
+//   Mie phase/amplitude lookup (scatterers/mie.jl:46-92); Spline phase/
+//   amplitude lookup (scatterers/spline.jl:26-76, NSPL = Nθ - 1).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -142,18 +143,22 @@ template <typename R> MCRT_DEV void rotate(R I[4], R c2, R s2) {
 
 // ------------------------------------------------------------ scatterers
 // Device view of a scatterer's tables.  F holds (F11, F12, F33, F34) per node
-// and S holds (S11 re, S11 im, S22 re, S22 im) per node, both interleaved.
+// (Mie) or (F11, F12, F22, F33, F34, F44, 0, 0) per node (Spline); S holds
+// (S11 re, S11 im, S22 re, S22 im) per node; all interleaved.
 template <typename R> struct ScaView {
-  int kind;    // 0 Rayleigh, 1 Mie
-  int Nt;      // Mie table length
+  int kind;    // 0 Rayleigh, 1 Mie, 2 Spline
+  int Nt;      // table length
   int rp;      // random_points
   R scale;     // (Nt - 1)/π
   const R* csrn;
   const R* F;
   const R* S;
 };
+// Words per node of a tabulated scatterer's F table.
+MCRT_HD int f_stride(int kind) { return kind == 2 ? 8 : 4; }
 
-// Inverse-CDF sample of cosθ from the uniform u (rayleigh.jl:9-11, mie.jl:47-51).
+// Inverse-CDF sample of cosθ from the uniform u (rayleigh.jl:9-11, mie.jl:47-51,
+// spline.jl:27-31: the same csrn lookup for Mie and Spline).
 // The lookups below keep a*b+c unfused: a fused `(i0 + 1) - u·rp` or
 // `(i0 + 1) - θ/π·(Nθ-1)` would pair an unrounded coordinate in one weight with
 // the rounded one in the other, so the two weights would not sum to 1 (by up to
@@ -172,33 +177,53 @@ template <typename R> MCRT_DEV R phase_sample(const ScaView<R>& s, R u) {
   return (R(mrn + 1) - rn) * s.csrn[mrn] + (rn - R(mrn)) * s.csrn[mrn + 1];
 }
 
-// Phase matrix at cosθ = mu: P = (P11, P12, P33, P34) (rayleigh.jl:12-17,
-// mie.jl:53-71).  The Mie branch clamps |mu| ≤ 1 before acos.
-template <typename R> MCRT_DEV void phase_eval(const ScaView<R>& s, R mu, R P[4]) {
+// Phase matrix at cosθ = mu: P = (P11, P12, P22, P33, P34, P44) (rayleigh.jl:12-17,
+// mie.jl:53-71 and spline.jl:33-57; P22 = P11 and P44 = P33 for Rayleigh and
+// Mie).  The table lookups clamp |mu| ≤ 1 before acos.  Mie interpolates
+// linearly with weights (msn - sn, sn - msn + 1); Spline keeps its weights
+// (msn + 1 - sn, sn - msn) as written, one node step further on.
+template <typename R> MCRT_DEV void phase_eval(const ScaView<R>& s, R mu, R P[6]) {
 #pragma clang fp contract(off)
   if (s.kind == 0) {
     R m2 = mu * mu;
     P[0] = R(0.75) * (R(1) + m2);
     P[1] = R(-0.75) * (R(1) - m2);
-    P[2] = R(1.5) * mu;
-    P[3] = R(0);
+    P[3] = R(1.5) * mu;
+    P[4] = R(0);
+    P[2] = P[0];
+    P[5] = P[3];
     return;
   }
   R th = M<R>::acos(clamp1(mu));
   R sn = th / M<R>::pi * R(s.Nt - 1);
   int i0 = (int)M<R>::trunc(sn);
   i0 = i0 < 0 ? 0 : (i0 > s.Nt - 1 ? s.Nt - 1 : i0);
-  R w0 = R(i0 + 1) - sn;
-  const R* a = s.F + 4 * i0;
-  P[0] = w0 * a[0]; P[1] = w0 * a[1]; P[2] = w0 * a[2]; P[3] = w0 * a[3];
+  if (s.kind == 1) {
+    R w0 = R(i0 + 1) - sn;
+    const R* a = s.F + 4 * i0;
+    P[0] = w0 * a[0]; P[1] = w0 * a[1]; P[3] = w0 * a[2]; P[4] = w0 * a[3];
+    if (i0 + 1 <= s.Nt - 1) {
+      R w1 = sn - R(i0 + 1) + R(1);
+      P[0] += w1 * a[4]; P[1] += w1 * a[5]; P[3] += w1 * a[6]; P[4] += w1 * a[7];
+    }
+    P[2] = P[0];
+    P[5] = P[3];
+    return;
+  }
+  R w0 = R(i0 + 2) - sn;
+  const R* a = s.F + 8 * i0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) P[c] = w0 * a[c];
   if (i0 + 1 <= s.Nt - 1) {
-    R w1 = sn - R(i0 + 1) + R(1);
-    P[0] += w1 * a[4]; P[1] += w1 * a[5]; P[2] += w1 * a[6]; P[3] += w1 * a[7];
+    R w1 = sn - R(i0 + 1);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) P[c] += w1 * a[8 + c];
   }
 }
 
 // Amplitude matrix diag(S11, S22) at cosθ = mu (rayleigh.jl:20-23,
-// mie.jl:76-92 with its one-step-shifted weights (msn+1-sn, sn-msn)).
+// mie.jl:76-92 and spline.jl:60-76, the same lookup with its one-step-shifted
+// weights (msn+1-sn, sn-msn)).
 template <typename R> MCRT_DEV void amplitude(const ScaView<R>& s, R mu, Cx<R>& S11, Cx<R>& S22) {
 #pragma clang fp contract(off)
   if (s.kind == 0) {
@@ -234,14 +259,14 @@ MCRT_DEV void update_I(const ScaView<R>& s, const R I[4], V3<R> k, Fr<R> f, R ou
   V3<R> Evk = ray_to_norm(V3<R>{ks.ct * ks.cp, ks.ct * ks.sp, -ks.st}, f);
   R cpsi = Eh.x * Ehk.x + Eh.y * Ehk.y;
   R spsi = -Eh.x * Evk.x - Eh.y * Evk.y;
-  R P[4];
+  R P[6];
   phase_eval(s, kr.z, P);
   R Ir[4] = {I[0], I[1], I[2], I[3]};
   rotate(Ir, ks.cp * ks.cp - ks.sp * ks.sp, R(-2) * ks.sp * ks.cp);
   out[0] = P[0] * Ir[0] + P[1] * Ir[1];
-  out[1] = P[1] * Ir[0] + P[0] * Ir[1];
-  out[2] = P[2] * Ir[2] + P[3] * Ir[3];
-  out[3] = -P[3] * Ir[2] + P[2] * Ir[3];
+  out[1] = P[1] * Ir[0] + P[2] * Ir[1];
+  out[2] = P[3] * Ir[2] + P[4] * Ir[3];
+  out[3] = -P[4] * Ir[2] + P[5] * Ir[3];
   rotate(out, cpsi * cpsi - spsi * spsi, R(2) * spsi * cpsi);
   if (Eh_out) *Eh_out = Eh;
   if (Ev_out) *Ev_out = Ev;

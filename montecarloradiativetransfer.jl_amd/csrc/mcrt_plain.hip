@@ -209,7 +209,7 @@ MCRT_DEV void collect_cb(const Ctx& c, const R I[4], V3<R> k, V3<R> x, int idx, 
 MCRT_DEV void scatter(const ScaView<R>& s, const R I[4], V3<R> k, Rng& rng, R Io[4], V3<R>& ko) {
   Fr<R> ks = sph(k);
   R ct = phase_sample(s, rng.u64());
-  R P[4];
+  R P[6];
   phase_eval(s, ct, P);
   R eq = -P[1] * I[1] / (P[0] * I[0]);
   R eu = -P[1] * I[2] / (P[0] * I[0]);
@@ -385,9 +385,18 @@ __global__ void plain_eval_kernel(const PlainParams* __restrict__ pp, int what, 
       r[1] = r[0];
       break;
     case MCRT_EVAL_PHASE:
-    case MCRT_EVAL_PHASE_TRANSPORT:
-      phase_eval(sca, a[0], r);
+    case MCRT_EVAL_PHASE_TRANSPORT: {
+      R P[6];
+      phase_eval(sca, a[0], P);
+      r[0] = P[0]; r[1] = P[1]; r[2] = P[3]; r[3] = P[4];
       break;
+    }
+    case MCRT_EVAL_PHASE_DIAG: {
+      R P[6];
+      phase_eval(sca, a[0], P);
+      r[0] = P[2]; r[1] = P[5]; r[2] = P[2]; r[3] = P[5];
+      break;
+    }
     case MCRT_EVAL_AMPLITUDE: {
       Cx<R> S11, S22;
       amplitude(sca, a[0], S11, S22);

@@ -40,7 +40,7 @@ namespace {
 // ---------------------------------------------------------- owned requests
 struct OwnedScatterer {
   mcrt_scatterer s{};
-  std::vector<double> csrn, F11, F12, F33, F34, S11, S22;
+  std::vector<double> csrn, F11, F12, F33, F34, S11, S22, F22, F44;
 };
 
 struct OwnedRequest {
@@ -65,7 +65,7 @@ const mcrt_scatterer* own_scatterer(OwnedRequest& o, const mcrt_scatterer* s,
   if (it != seen.end()) return it->second;
   auto c = std::make_unique<OwnedScatterer>();
   c->s = *s;
-  if (s->kind == MCRT_SCATTERER_MIE) {
+  if (s->kind == MCRT_SCATTERER_MIE || s->kind == MCRT_SCATTERER_SPLINE) {
     const int64_t N = s->n_theta;
     c->s.csrn = keep(c->csrn, s->csrn, (int64_t)s->random_points + 1);
     c->s.F11 = keep(c->F11, s->F11, N);
@@ -74,6 +74,10 @@ const mcrt_scatterer* own_scatterer(OwnedRequest& o, const mcrt_scatterer* s,
     c->s.F34 = keep(c->F34, s->F34, N);
     c->s.S11 = keep(c->S11, s->S11, 2 * N);
     c->s.S22 = keep(c->S22, s->S22, 2 * N);
+    if (s->kind == MCRT_SCATTERER_SPLINE) {
+      c->s.F22 = keep(c->F22, s->F22, N);
+      c->s.F44 = keep(c->F44, s->F44, N);
+    }
   }
   const mcrt_scatterer* r = &c->s;
   o.sca.push_back(std::move(c));

@@ -26,6 +26,7 @@ MCRT_ERR_NO_DEVICE = -7
 
 SCATTERER_RAYLEIGH = 0
 SCATTERER_MIE = 1
+SCATTERER_SPLINE = 2
 GEOMETRY_PLANE = 0
 GEOMETRY_SPHERE = 1
 
@@ -52,6 +53,8 @@ class mcrt_scatterer(C.Structure):
         ("F34", _dp),
         ("S11", _dp),
         ("S22", _dp),
+        ("F22", _dp),
+        ("F44", _dp),
     ]
 
 
@@ -143,7 +146,8 @@ class mcrt_request(C.Structure):
 
 
 class mcrt_launch_shape(C.Structure):
-    _fields_ = [("slots", C.c_int32), ("workgroups_per_cu", C.c_int32)]
+    _fields_ = [("slots", C.c_int32), ("workgroups_per_cu", C.c_int32), ("stage_s", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class mcrt_server_stats(C.Structure):
@@ -165,7 +169,7 @@ class mcrt_server_stats(C.Structure):
 
 # mcrt_plan_eval functions (include/mcrt.h MCRT_EVAL_*)
 EVAL = {"update_I": 0, "update_E": 1, "update_E_path": 2, "distance": 3, "phase": 4,
-        "amplitude": 5, "sample": 6, "phase_transport": 7, "kepler": 8}
+        "amplitude": 5, "sample": 6, "phase_transport": 7, "kepler": 8, "phase_diag": 9}
 
 TICKET_STATES = {0: "queued", 1: "running", 2: "done", 3: "cancelled", 4: "failed"}
 

@@ -127,7 +127,7 @@ class Plan:
         h = C.c_void_p()
         if ray_begin < 0 or not 0 <= instance < 2**32:
             raise ValueError("ray_begin must be >= 0 and instance a 32-bit unsigned integer")
-        sh = _abi.mcrt_launch_shape(*(shape or (0, 0)))
+        sh = _abi.mcrt_launch_shape(*(tuple(shape or ()) + (0, 0, 0))[:3])
         rc = self.ctx.lib.mcrt_plan_create_shaped(self.ctx.handle, C.byref(cs), C.byref(gs),
                                                   _abi.dptr(i0), C.c_uint64(seed & (2**64 - 1)),
                                                   _abi.PRECISIONS[precision], C.c_uint64(ray_begin),
@@ -136,13 +136,14 @@ class Plan:
         self.handle = h
 
     def shape(self) -> dict:
-        """Launch shape: slots per workgroup, workgroups per CU, grid size."""
+        """Launch shape: slots per workgroup, workgroups per CU, where S is
+        read (1 LDS, -1 HBM), grid size."""
         sh = _abi.mcrt_launch_shape()
         wg = C.c_int32()
         _abi.check(self.ctx.lib.mcrt_plan_shape(self.handle, C.byref(sh), C.byref(wg)),
                    self.ctx.handle)
         return {"slots": sh.slots, "workgroups_per_cu": sh.workgroups_per_cu,
-                "workgroups": wg.value}
+                "stage_s": sh.stage_s, "workgroups": wg.value}
 
     def launch(self) -> None:
         _abi.check(self.ctx.lib.mcrt_plan_launch(self.handle), self.ctx.handle)

@@ -16,8 +16,10 @@ index arithmetic count 0; dense StaticArrays products count their zero terms
   dense 4×4·4          utils.jl:108      16 mul + 12 add                        =  28
   phase matrix         rayleigh.jl:12-17 μ², 1±μ² (2), ×0.75 (2), 1.5μ         =   6
                        mie.jl:53-71      acos, /π, ×(N-1), w 3, 8 mul, 4 add    =  18
+                       spline.jl:33-50   acos, /π, ×NSPL, w 3, 12 mul, 6 add    =  24
   amplitude matrix     rayleigh.jl:21    μ·1.0im                                 =   2
                        mie.jl:77-86      acos, /π, ×(N-1), w 3, 8 mul, 4 add    =  18
+                       spline.jl:61-70   (as Mie)                                =  18
   update_E             utils.jl:170-192  frame 77 + ϕ-rotation 12 + S·E (4 cmul
                                          24 + 2 cadd 4) + ψ-rotation 12 + S      = 129 + S
   I_from_E             utils.jl:157-163  3 + 3 + 2 cmul 12 + 2 + 2 + cmul 6 + 2 =  30
@@ -35,7 +37,8 @@ index arithmetic count 0; dense StaticArrays products count their zero terms
                                          + 4 + 34 + 6 + 5 + 6 + 4
 
 Per-event transport (scatter, free path: ~300 flops) is < 0.1% of the bin
-work at these workloads and is not counted.
+work at these workloads and is not counted.  A plane whose layers hold
+different scatterers is counted with layer 1's.
 """
 from __future__ import annotations
 
@@ -46,8 +49,8 @@ SPH = 10
 FRAME = GET_BASIS + NORM_TO_RAY + SPH + RAY_TO_NORM + (2 + RAY_TO_NORM) + 6  # 77
 ROTATES = 22
 DENSE = 28
-PHASE = {"rayleigh": 6, "mie": 18}
-AMPL = {"rayleigh": 2, "mie": 18}
+PHASE = {"rayleigh": 6, "mie": 18, "spline": 24}
+AMPL = {"rayleigh": 2, "mie": 18, "spline": 18}
 I_FROM_E = 30
 DIST = {"plane": 2, "sphere": 16}
 
@@ -79,10 +82,12 @@ def f_cb1(geom: str, sca: str) -> int:
 
 def kinds(g) -> tuple[str, str]:
     from .geometries import PlaneGeometry
-    from .scatterers import RayleighScatterer
+    from .scatterers import RayleighScatterer, SplineScatterer
     geom = "plane" if isinstance(g, PlaneGeometry) else "sphere"
     sca = g.layers[0].scatterer if geom == "plane" else g.scatterer
-    return geom, ("rayleigh" if isinstance(sca, RayleighScatterer) else "mie")
+    if isinstance(sca, RayleighScatterer):
+        return geom, "rayleigh"
+    return geom, ("spline" if isinstance(sca, SplineScatterer) else "mie")
 
 
 def algorithmic_flops(g, stats: dict) -> float:

@@ -1,11 +1,12 @@
-"""Scatterers: RayleighScatterer and MieScatterer.
+"""Scatterers: RayleighScatterer, MieScatterer and SplineScatterer.
 
 Mirrors src/scatterers/index.jl (mean_free_path, single_scattering_albedo),
-src/scatterers/rayleigh.jl:3-6 and src/scatterers/mie.jl:3-44.  The Mie
-tables come from the library's FP64 builders (``mcrt_bhmie``,
-``mcrt_calculate_csrn``), the same arithmetic the device tables are staged
-from.  SplineScatterer (src/scatterers/spline.jl) is not provided: its
-reference methods read the undefined ``self.NSPL`` and throw on first use.
+src/scatterers/rayleigh.jl:3-6, src/scatterers/mie.jl:3-44 and
+src/scatterers/spline.jl:3-24.  The Mie tables come from the library's FP64
+builders (``mcrt_bhmie``, ``mcrt_calculate_csrn``), the same arithmetic the
+device tables are staged from.  SplineScatterer's lookups read ``self.NSPL``
+in the reference, a name defined nowhere (every call throws); the library
+reads it as Nθ - 1, the table's last node, and keeps the rest as written.
 """
 from __future__ import annotations
 
@@ -129,6 +130,50 @@ class MieScatterer(AbstractScatterer):
                                         "asymm": res.asymm})
 
 
+@dataclass(frozen=True, eq=False)
+class SplineScatterer(AbstractScatterer):
+    """SplineScatterer — src/scatterers/spline.jl:3-15 (struct) and :19-24
+    (constructor): user tables F11, F12, F22, F33, F34, F44, S11, S22 of one
+    length Nθ on θ = range(0, π, Nθ), ω and kl given, csrn built from F11.
+
+    Build it with ``SplineScatterer.create(ω, kl, F11, F12, F22, F33, F34,
+    F44, S11, S22; random_points)`` (the reference's positional constructor),
+    or pass the fields directly.  The lookups (spline.jl:26-76) take
+    NSPL = Nθ - 1; the phase matrix keeps the reference's weights
+    (msn+1-sn, sn-msn), one node step on from a linear interpolation.
+    """
+
+    omega: float
+    kl: float
+    csrn: np.ndarray
+    F11: np.ndarray
+    F12: np.ndarray
+    F22: np.ndarray
+    F33: np.ndarray
+    F34: np.ndarray
+    F44: np.ndarray
+    S11: np.ndarray  # complex128
+    S22: np.ndarray
+
+    @classmethod
+    def create(cls, omega: float, kl: float, F11, F12, F22, F33, F34, F44, S11, S22,
+               random_points: int = 1000) -> "SplineScatterer":
+        F = [_abi.f64(f) for f in (F11, F12, F22, F33, F34, F44)]
+        S = [np.asarray(z, dtype=np.complex128) for z in (S11, S22)]
+        if any(len(a) != len(F[0]) for a in F + S):
+            raise ValueError("the Spline tables must have one length")
+        csrn = calculate_csrn(F[0], random_points)
+        return cls(float(omega), float(kl), csrn, *F, *S)
+
+    @property
+    def Ntheta(self) -> int:
+        return len(self.F11)
+
+    @property
+    def random_points(self) -> int:
+        return len(self.csrn) - 1
+
+
 def to_struct(s: AbstractScatterer, keep: list) -> _abi.mcrt_scatterer:
     """Fill an mcrt_scatterer; arrays referenced by it are appended to keep."""
     st = _abi.mcrt_scatterer()
@@ -137,19 +182,24 @@ def to_struct(s: AbstractScatterer, keep: list) -> _abi.mcrt_scatterer:
     if isinstance(s, RayleighScatterer):
         st.kind = _abi.SCATTERER_RAYLEIGH
         return st
-    if not isinstance(s, MieScatterer):
+    if isinstance(s, MieScatterer):
+        st.kind = _abi.SCATTERER_MIE
+        names = ("csrn", "F11", "F12", "F33", "F34")
+    elif isinstance(s, SplineScatterer):
+        st.kind = _abi.SCATTERER_SPLINE
+        names = ("csrn", "F11", "F12", "F22", "F33", "F34", "F44")
+    else:
         raise TypeError(f"unsupported scatterer type {type(s).__name__}")
-    st.kind = _abi.SCATTERER_MIE
     st.n_theta = int(s.Ntheta)
     st.random_points = int(s.random_points)
-    arrs = {k: _abi.f64(getattr(s, k)) for k in ("csrn", "F11", "F12", "F33", "F34")}
+    arrs = {k: _abi.f64(getattr(s, k)) for k in names}
     for k in ("S11", "S22"):
         z = np.asarray(getattr(s, k), dtype=np.complex128)
         arrs[k] = np.ascontiguousarray(np.stack([z.real, z.imag], axis=1).reshape(-1))
     if len(arrs["csrn"]) != st.random_points + 1:
         raise ValueError("csrn must hold random_points + 1 values")
-    for k in ("F11", "F12", "F33", "F34"):
-        if len(arrs[k]) != st.n_theta:
+    for k in names[1:] + ("S11", "S22"):
+        if len(arrs[k]) != st.n_theta * (2 if k[0] == "S" else 1):
             raise ValueError(f"{k} must hold Nθ values")
     for k, a in arrs.items():
         keep.append(a)

@@ -9,6 +9,11 @@
   plane_rayleigh_cb     plane_rayleigh_tau1 with CB on (default θb grid), 1e9 rays:
                         the mixed-vs-FP64 precision case
   sphere_mie_rt         sphere_mie_cb without CB
+  sphere_spline_cb      sphere_mie_cb with a SplineScatterer on the same tables,
+                        P22 = 0.93·P11 and P44 = 0.97·P33 (a depolarising
+                        scatterer the Mie form cannot express)
+  plane_rayleigh_over_mie  two layers: Rayleigh τ = 0.5 over the C2 Mie
+                        scatterer τ = 1.5, normal incidence, CB on
 The ray counts of the full workloads are 1e6, 1e7 and 1e9; tests use fewer rays
 of the same configuration.
 """
@@ -20,7 +25,7 @@ import numpy as np
 
 from .config import Config, midpoint_polar_grid
 from .geometries import Layer, PlaneGeometry, SphereGeometry
-from .scatterers import MieScatterer, RayleighScatterer
+from .scatterers import MieScatterer, RayleighScatterer, SplineScatterer
 
 _MIE_CACHE: dict = {}
 
@@ -30,6 +35,17 @@ def c2_mie(Ntheta: int = 1801) -> MieScatterer:
     if key not in _MIE_CACHE:
         _MIE_CACHE[key] = MieScatterer.create(0.5, 1.5 + 0.001j, 0.55, 0.1, Ntheta=Ntheta,
                                               random_points=1000)
+    return _MIE_CACHE[key]
+
+
+def spline_from(s: MieScatterer, f22: float = 0.93, f44: float = 0.97) -> SplineScatterer:
+    """A SplineScatterer on a Mie scatterer's tables with P22 = f22·P11 and
+    P44 = f44·P33 (ω, kl and the amplitude tables as the Mie scatterer's)."""
+    key = ("spline", id(s), f22, f44)
+    if key not in _MIE_CACHE:
+        _MIE_CACHE[key] = SplineScatterer.create(s.omega, s.kl, s.F11, s.F12, f22 * s.F11, s.F33,
+                                                 s.F34, f44 * s.F33, s.S11, s.S22,
+                                                 random_points=s.random_points)
     return _MIE_CACHE[key]
 
 
@@ -75,5 +91,32 @@ def sphere_mie_rt(n_rays: int = 10**7):
     return sphere_mie_cb(n_rays, compute_cb=False)
 
 
+def sphere_spline_cb(n_rays: int = 10**7, *, tau: float = 5.0):
+    """sphere_mie_cb with a SplineScatterer on the C2 tables (spline_from)."""
+    cfg, g, seed = sphere_mie_cb(n_rays, tau=tau)
+    sp = spline_from(c2_mie(1801))
+    g2 = SphereGeometry(R=tau * sp.kl, scatterer=sp)
+    return cfg.replace(geometry=g2), g2, seed
+
+
+def plane_rayleigh_over_mie(n_rays: int = 10**6, *, tau_top: float = 0.5, tau_mie: float = 1.5,
+                            compute_cb: bool = True, Ntheta: int = 180, Nphi: int = 48):
+    """Two layers with different scatterers: Rayleigh (kl = 1) over the C2 Mie
+    scatterer, normal incidence.  The peel-off's reflected range uses the
+    Rayleigh layer, the transmitted range the Mie layer, CB layer 1's
+    scatterer (cpu.jl:50)."""
+    ray = RayleighScatterer(1.0, 1.0)
+    mie = c2_mie(1801)
+    g = PlaneGeometry(math.pi, 0.0, [Layer(d=tau_top * ray.kl, scatterer=ray, tau=tau_top, omega=1.0),
+                                     Layer(d=tau_mie * mie.kl, scatterer=mie, tau=tau_mie,
+                                           omega=mie.omega)])
+    ct, st, w = midpoint_polar_grid(g, Ntheta, Nphi)
+    cfg = Config(g, number_of_rays=n_rays, maximum_scattering_times=10000, compute_cb=compute_cb,
+                 Ntheta=Ntheta, Nphi=Nphi, costheta=ct, sintheta=st, inorm=w,
+                 minimum_intensity=1e-6, tau0=50.0)
+    return cfg, g, 11
+
+
 WORKLOADS = {"plane_rayleigh_tau1": plane_rayleigh_tau1, "sphere_mie_cb": sphere_mie_cb,
-             "plane_rayleigh_cb": plane_rayleigh_cb, "sphere_mie_rt": sphere_mie_rt}
+             "plane_rayleigh_cb": plane_rayleigh_cb, "sphere_mie_rt": sphere_mie_rt,
+             "sphere_spline_cb": sphere_spline_cb, "plane_rayleigh_over_mie": plane_rayleigh_over_mie}

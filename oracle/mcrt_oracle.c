@@ -178,7 +178,8 @@ void or_rotate(const double I[4], double c2, double s2, double o[4]) {
 /* Scatterers — src/scatterers/rayleigh.jl, mie.jl                           */
 /* ------------------------------------------------------------------------ */
 /* Inverse-CDF sample of cosθ given the uniform u:
- * Rayleigh rayleigh.jl:9-11 (Cardano), Mie mie.jl:47-51 (csrn lerp). */
+ * Rayleigh rayleigh.jl:9-11 (Cardano), Mie mie.jl:47-51 and Spline
+ * spline.jl:27-31 (csrn lerp; length(csrn) - 1 = random_points). */
 double or_phase_sample(const mcrt_scatterer* s, double u) {
   if (s->kind == MCRT_SCATTERER_RAYLEIGH) {
     double rn = 2.0 * (1.0 - 2.0 * u);
@@ -192,16 +193,21 @@ double or_phase_sample(const mcrt_scatterer* s, double u) {
   }
 }
 
-/* phase_matrix at a given cosθ: Rayleigh rayleigh.jl:12-17, Mie mie.jl:53-71.
- * P[0..3] = (P11, P12, P33, P34); P22 = P11, P44 = P33, P43 = -P34. */
-void or_phase_eval(const mcrt_scatterer* s, double mu, double P[4]) {
+/* phase_matrix at a given cosθ: Rayleigh rayleigh.jl:12-17, Mie mie.jl:53-71,
+ * Spline spline.jl:26-58 with NSPL read as Nθ - 1 (the name is undefined in
+ * the reference) and its weights (msn+1-sn, sn-msn) as written.
+ * P[0..5] = (P11, P12, P22, P33, P34, P44); P21 = P12, P43 = -P34; Rayleigh
+ * and Mie have P22 = P11, P44 = P33. */
+void or_phase_eval(const mcrt_scatterer* s, double mu, double P[6]) {
   if (s->kind == MCRT_SCATTERER_RAYLEIGH) {
     double m2 = mu * mu;
     P[0] = 0.75 * (1.0 + m2);
     P[1] = -0.75 * (1.0 - m2);
-    P[2] = 1.5 * mu;
-    P[3] = 0.0;
-  } else {
+    P[3] = 1.5 * mu;
+    P[4] = 0.0;
+    P[2] = P[0];
+    P[5] = P[3];
+  } else if (s->kind == MCRT_SCATTERER_MIE) {
     int N = s->n_theta;
     double th = acos(clamp1(mu));
     double sn = th / OR_PI * (double)(N - 1);
@@ -218,12 +224,29 @@ void or_phase_eval(const mcrt_scatterer* s, double mu, double P[4]) {
       F33 += w1 * s->F33[i0 + 1];
       F34 += w1 * s->F34[i0 + 1];
     }
-    P[0] = F11; P[1] = F12; P[2] = F33; P[3] = F34;
+    P[0] = F11; P[1] = F12; P[2] = F11; P[3] = F33; P[4] = F34; P[5] = F33;
+  } else {
+    /* spline.jl:33-50: sn = θ/π·NSPL, msn = trunc(sn) + 1 (1-based), first
+     * term (msn+1-sn)·F[msn], second (sn-msn)·F[msn+1] while msn+1 ≤ length */
+    int N = s->n_theta;
+    double th = acos(clamp1(mu));
+    double sn = th / OR_PI * (double)(N - 1);
+    long i0 = (long)trunc(sn);
+    if (i0 > N - 1) i0 = N - 1;
+    if (i0 < 0) i0 = 0;
+    const double* T[6] = {s->F11, s->F12, s->F22, s->F33, s->F34, s->F44};
+    double w0 = (double)(i0 + 2) - sn;
+    for (int c = 0; c < 6; ++c) P[c] = w0 * T[c][i0];
+    if (i0 + 1 <= N - 1) {
+      double w1 = sn - (double)(i0 + 1);
+      for (int c = 0; c < 6; ++c) P[c] += w1 * T[c][i0 + 1];
+    }
   }
 }
 
-/* amplitude_matrix: Rayleigh rayleigh.jl:20-23; Mie mie.jl:76-92 including
- * its one-step-shifted interpolation weights (msn+1-sn, sn-msn). */
+/* amplitude_matrix: Rayleigh rayleigh.jl:20-23; Mie mie.jl:76-92 and Spline
+ * spline.jl:60-76 (the same lookup, NSPL read as Nθ - 1) including its
+ * one-step-shifted interpolation weights (msn+1-sn, sn-msn). */
 void or_amplitude(const mcrt_scatterer* s, double mu, or_cplx S[2]) {
   if (s->kind == MCRT_SCATTERER_RAYLEIGH) {
     S[0] = cmk(0.0, 1.0);
@@ -265,15 +288,15 @@ void or_update_I(const mcrt_scatterer* s, const double I[4], const double k[3],
   or_ray_to_norm(v2, f, Evk);
   double cpsi = Eh[0] * Ehk[0] + Eh[1] * Ehk[1];
   double spsi = -Eh[0] * Evk[0] - Eh[1] * Evk[1];
-  double P[4];
+  double P[6];
   or_phase_eval(s, kr[2], P);
   double Ir[4];
   or_rotate(I, ks[2] * ks[2] - ks[3] * ks[3], -2.0 * ks[3] * ks[2], Ir);
   double Ip[4];
   Ip[0] = P[0] * Ir[0] + P[1] * Ir[1];
-  Ip[1] = P[1] * Ir[0] + P[0] * Ir[1];
-  Ip[2] = P[2] * Ir[2] + P[3] * Ir[3];
-  Ip[3] = -P[3] * Ir[2] + P[2] * Ir[3];
+  Ip[1] = P[1] * Ir[0] + P[2] * Ir[1];
+  Ip[2] = P[3] * Ir[2] + P[4] * Ir[3];
+  Ip[3] = -P[4] * Ir[2] + P[5] * Ir[3];
   or_rotate(Ip, cpsi * cpsi - spsi * spsi, 2.0 * spsi * cpsi, out);
 }
 
@@ -746,7 +769,7 @@ static void or_scatter(const geo_t* G, const double I[4], const double k[3], int
   double ks[4];
   or_spherical_from_cartesian(k, ks);
   double ct = or_phase_sample(s, rnd(rng));
-  double P[4];
+  double P[6];
   or_phase_eval(s, ct, P);
   double eq = -P[1] * I[1] / (P[0] * I[0]);
   double eu = -P[1] * I[2] / (P[0] * I[0]);

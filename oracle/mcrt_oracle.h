@@ -3,7 +3,7 @@
  *
  * CPU FP64 restatement of the reference's per-ray transport
  * (src/cpu.jl, src/utils.jl, src/geometries/{plane,sphere}.jl, src/scatterers/{index,
- * rayleigh,mie}.jl, src/config.jl, src/MonteCarloRadiativeTransfer.jl:26-85),
+ * rayleigh,mie,spline}.jl, src/config.jl, src/MonteCarloRadiativeTransfer.jl:26-85),
  * with Julia's rand() replaced by a counter-based Philox4x32-10 stream.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it, and only as the checker; the product (libmcrt_hip.so) never does.
@@ -46,8 +46,9 @@ double or_backward_E(const mcrt_scatterer* s, const or_cplx E[2], const double* 
                      int64_t n, const double f0[4], or_cplx out[2]);
 double or_kepler_solver(double M, double e);
 
-/* Scatterers.  phase_matrix evaluation at a given cosθ: P = (F11, F12, F33, F34). */
-void or_phase_eval(const mcrt_scatterer* s, double mu, double P[4]);
+/* Scatterers.  phase_matrix evaluation at a given cosθ:
+ * P = (P11, P12, P22, P33, P34, P44). */
+void or_phase_eval(const mcrt_scatterer* s, double mu, double P[6]);
 double or_phase_sample(const mcrt_scatterer* s, double u);
 void or_amplitude(const mcrt_scatterer* s, double mu, or_cplx S[2]);
 

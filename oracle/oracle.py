@@ -158,7 +158,8 @@ class Sca:
 
 
 def phase_eval(sca: Sca, mu):
-    P = np.zeros(4)
+    """(P11, P12, P22, P33, P34, P44) of phase_matrix(s, μ)."""
+    P = np.zeros(6)
     lib().or_phase_eval(sca.ptr, mu, P.ctypes.data_as(_dp))
     return P
 

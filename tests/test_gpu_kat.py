@@ -7,7 +7,8 @@ restatement of the reference functions:
   update_I (utils.jl:91-112) as the peel-off evaluates it, update_E
   (utils.jl:170-192) as the CB bins and as the FP64 CB path state evaluate it,
   distance_to_boundary (plane.jl:97-105, sphere.jl:69-73), the phase and
-  amplitude lookups (rayleigh.jl:8-23, mie.jl:46-92), inverse-CDF sampling and
+  amplitude lookups (rayleigh.jl:8-23, mie.jl:46-92, spline.jl:26-76 with
+  NSPL = Nθ - 1, including P22 and P44), inverse-CDF sampling and
   kepler_solver (utils.jl:243-261).
 
 Bounds: 1e-12 relative to the output's scale in the FP64 forms (fp64_plain
@@ -17,11 +18,12 @@ in FP32: its bound is FP32_REL (1e-5) of the output's scale, where the scale is
 max(1, max |reference|) — the FP32 arccos of the table lookup (|error| ≤ 2e-8
 rad, Abramowitz & Stegun 4.4.46) and the FP32 rounding of μ (≈ 6e-8/sinθ rad,
 ≲ 2.5e-4 of a 0.1° table step for |μ| ≤ 0.99) dominate; distances are bounded
-at FP32_REL of the medium's size.  Inputs keep |μ| ≤ 0.99 for Mie lookups,
+at FP32_REL of the larger of the medium's size and the distance.  Inputs keep |μ| ≤ 0.99 for Mie lookups,
 where the lookup is well conditioned; the polar branch (μ = ±1 exactly) is
 exercised with Rayleigh.  At the table nodes themselves the reference's
 amplitude lookup is discontinuous (its shifted weights), and either one-sided
-value is accepted there.
+value is accepted there, and so it is for the Spline phase lookup, which has
+the same weights.
 """
 from __future__ import annotations
 
@@ -60,8 +62,16 @@ def _kdir(f):
     return np.array([f[1] * f[2], f[1] * f[3], f[0]])
 
 
+def _scatterer(kind):
+    if kind == "rayleigh":
+        return mcrt.RayleighScatterer(0.9, 1.3)
+    if kind == "spline":  # P22 = 0.93·P11, P44 = 0.97·P33 on the Mie tables
+        return mcrt.workloads.spline_from(mcrt.workloads.c2_mie(1801))
+    return mcrt.workloads.c2_mie(1801)
+
+
 def _case(kind, geom):
-    sca = mcrt.RayleighScatterer(0.9, 1.3) if kind == "rayleigh" else mcrt.workloads.c2_mie(1801)
+    sca = _scatterer(kind)
     if geom == "plane":
         g = mcrt.PlaneGeometry(0.8 * math.pi, 0.4, [mcrt.Layer(d=2.0 * sca.kl, scatterer=sca, tau=2.0),
                                                       mcrt.Layer(d=1.0 * sca.kl, scatterer=sca, tau=1.0)])
@@ -103,7 +113,7 @@ def _check(form, got, want, label, scale=None, inputs=None):
 
 
 @pytest.mark.parametrize("form", FORMS)
-@pytest.mark.parametrize("kind", ["rayleigh", "mie"])
+@pytest.mark.parametrize("kind", ["rayleigh", "mie", "spline"])
 def test_kat_update_I_and_E(form, kind):
     sca, g, cfg = _case(kind, "sphere")
     osca = oracle.Sca(sca)
@@ -111,7 +121,7 @@ def test_kat_update_I_and_E(form, kind):
     try:
         n = 600
         fr = _frames(n)
-        ks = _directions(fr, kind == "mie", n)
+        ks = _directions(fr, kind != "rayleigh", n)
         I = np.concatenate([np.ones((n, 1)), _unit(n) * rng.uniform(0, 1, (n, 1))], 1) * \
             rng.uniform(0.1, 2.0, (n, 1))
         E = rng.normal(size=(n, 4))
@@ -155,7 +165,10 @@ def test_kat_distance(form, geom):
         want = np.array([S.distance(x[i], ks[i], int(idx[i])) for i in range(n)])
         fin = np.isfinite(want)
         assert np.array_equal(np.isfinite(got[:, 0]), fin)
-        _check(form, got[fin, :1], want[fin, None], f"{geom} distance", scale=size)
+        # FP32 keeps a distance to FP32_REL of itself: near-horizontal directions
+        # in a slab give distances of thousands of optical depths
+        _check(form, got[fin, :1], want[fin, None], f"{geom} distance",
+               scale=np.maximum(size, np.abs(want[fin])))
         # the CB path state's FP64 distance
         _check("fp64", got[fin, 1:2], want[fin, None], f"{geom} distance (FP64 path state, {form})",
                scale=size)
@@ -163,8 +176,24 @@ def test_kat_distance(form, geom):
         plan.close()
 
 
+def _either_side(got, want, mu, tab, Nt):
+    """The reference's weights (msn+1-sn, sn-msn) make a lookup jump at every
+    node j: T[j-1] just below, 2T[j] - T[j+1] at and above (mie.jl:76-86,
+    spline.jl:35-50).  At a node, rounding of arccos decides the side, so the
+    device may take either one-sided value: want takes the one nearer got."""
+    sn = np.arccos(mu) / math.pi * (Nt - 1)
+    node = np.abs(sn - np.round(sn)) < 1e-9
+    j = np.round(sn).astype(int)
+    want = want.copy()
+    for i in np.nonzero(node)[0]:
+        lo = tab[j[i] - 1]
+        hi = 2 * tab[j[i]] - tab[j[i] + 1]
+        want[i] = lo if np.abs(got[i] - lo).max() < np.abs(got[i] - hi).max() else hi
+    return want, int(node.sum())
+
+
 @pytest.mark.parametrize("form", FORMS)
-@pytest.mark.parametrize("kind", ["rayleigh", "mie"])
+@pytest.mark.parametrize("kind", ["rayleigh", "mie", "spline"])
 def test_kat_lookups_sampling_kepler(form, kind):
     sca, g, cfg = _case(kind, "sphere")
     osca = oracle.Sca(sca)
@@ -172,31 +201,37 @@ def test_kat_lookups_sampling_kepler(form, kind):
     try:
         n = 2000
         mu = rng.uniform(-0.99, 0.99, n)
-        if kind == "mie":  # table nodes and midpoints too
+        if kind != "rayleigh":  # table nodes and midpoints too
             th = (np.arange(0, 1800, 9) + rng.choice([0.0, 0.5], 200)) * math.pi / 1800
             mu = np.concatenate([mu, np.cos(th[np.abs(np.cos(th)) <= 0.99])])
         else:
             mu = np.concatenate([mu, [-1.0, 1.0, 0.0]])
-        P = np.array([oracle.phase_eval(osca, m) for m in mu])
-        _check(form, plan.eval("phase", mu[:, None]), P, f"{kind} phase (peel-off)", inputs=mu)
-        _check(form, plan.eval("phase_transport", mu[:, None]), P, f"{kind} phase (transport)",
-               inputs=mu)
+        P6 = np.array([oracle.phase_eval(osca, m) for m in mu])  # P11 P12 P22 P33 P34 P44
+        P = P6[:, [0, 1, 3, 4]]
+        D = P6[:, [2, 5, 2, 5]]
+        for what, want, label in (("phase", P, "phase (peel-off)"),
+                                  ("phase_transport", P, "phase (transport)"),
+                                  ("phase_diag", D, "P22, P44 (peel-off, transport)")):
+            got = plan.eval(what, mu[:, None])
+            if kind == "spline":
+                cols = {"phase": [0, 1, 3, 4], "phase_transport": [0, 1, 3, 4],
+                        "phase_diag": [2, 5, 2, 5]}[what]
+                tab = np.stack([sca.F11, sca.F12, sca.F22, sca.F33, sca.F34, sca.F44], 1)[:, cols]
+                # each lookup takes its own side of a node: phase_diag holds two
+                groups = [[0, 1], [2, 3]] if what == "phase_diag" else [[0, 1, 2, 3]]
+                want = want.copy()
+                for gcols in groups:
+                    w, nn = _either_side(got[:, gcols], want[:, gcols], mu, tab[:, gcols],
+                                         sca.Ntheta)
+                    want[:, gcols] = w
+                    assert nn > 50  # the node cases were exercised
+            _check(form, got, want, f"{kind} {label}", inputs=mu)
         S = np.array([[v.real, v.imag, w.real, w.imag] for v, w in (oracle.amplitude(osca, m) for m in mu)])
         got = plan.eval("amplitude", mu[:, None])
-        if kind == "mie":
-            # The reference's amplitude weights (msn+1-sn, sn-msn) make the lookup
-            # jump at every node j: S[j-1] just below, 2S[j] - S[j+1] at and above
-            # (mie.jl:76-86).  At a node, rounding of arccos decides the side, so
-            # the device may take either one-sided value.
-            sn = np.arccos(mu) / math.pi * (sca.Ntheta - 1)
-            node = np.abs(sn - np.round(sn)) < 1e-9
-            j = np.round(sn).astype(int)
+        if kind != "rayleigh":
             tab = np.stack([sca.S11.real, sca.S11.imag, sca.S22.real, sca.S22.imag], 1)
-            for i in np.nonzero(node)[0]:
-                lo = tab[j[i] - 1]
-                hi = 2 * tab[j[i]] - tab[j[i] + 1]
-                S[i] = lo if np.abs(got[i] - lo).max() < np.abs(got[i] - hi).max() else hi
-            assert node.sum() > 50  # the node cases were exercised
+            S, nn = _either_side(got, S, mu, tab, sca.Ntheta)
+            assert nn > 50  # the node cases were exercised
         _check(form, got, S, f"{kind} amplitude", inputs=mu)
         u = np.concatenate([rng.uniform(1e-6, 1 - 1e-6, 1000), [0.5, 0.25, 0.75]])
         want = np.array([oracle.phase_sample(osca, x) for x in u])

@@ -5,7 +5,8 @@ Rayleigh and Mie, optical thickness or radius 0.1 to 20, size parameter 0.5
 to 20, Im m 1e-5 to 1e-1, Re m 1.3 to 1.7), with CB on the population's
 grid (θb 0–5° every 0.05° plus 6–180° every 1°, × 8 azimuths) and 180 × 48
 peel-off bins, 512 to 2048 rays each, half of them in the mixed form and half
-in FP64.  Each solve has its own Philox stream (seed, instance).  Every batched
+in FP64; four more mix slabs whose layers hold different scatterers and
+SplineScatterer solves into a batch and a server.  Each solve has its own Philox stream (seed, instance).  Every batched
 result and every served result (all sixteen submitted at once, so they share
 launches) is compared with the oracle on that same stream, per bin, with σ
 from the oracle's 32 batch means: the |z| > 3 count under a binomial bound
@@ -115,6 +116,52 @@ def test_batched_population_corners_match_the_oracle():
         for i, r in enumerate(reqs):
             st = {}
             _judge(r, b.read(i, stats=st), st, "batch")
+
+
+def _layered_and_spline():
+    """Solves that need per-layer tables or the Spline lookup, beside two corners."""
+    mie = mcrt.MieScatterer.create(0.3, complex(1.45, 0.01), LAM, NU, Ntheta=721, random_points=1000)
+    sp = mcrt.workloads.spline_from(mie)
+    ray = mcrt.RayleighScatterer(1.0, 1.0)
+    geos = [
+        ("rayleigh-over-mie slab", mcrt.PlaneGeometry(math.pi, 0.0, [
+            mcrt.Layer(d=0.5, scatterer=ray, tau=0.5, omega=1.0),
+            mcrt.Layer(d=2.0 * mie.kl, scatterer=mie, tau=2.0, omega=mie.omega)])),
+        ("mie-over-semi-infinite-rayleigh slab", mcrt.PlaneGeometry(math.radians(140.0), 0.3, [
+            mcrt.Layer(d=1.0 * mie.kl, scatterer=mie, tau=1.0, omega=mie.omega),
+            mcrt.Layer(d=-1.0, scatterer=mcrt.RayleighScatterer(0.95, 1.0), tau=1.0, omega=0.95)])),
+        ("spline sphere", mcrt.SphereGeometry(R=3.0 * sp.kl, scatterer=sp)),
+        ("rayleigh-over-spline slab", mcrt.PlaneGeometry(math.pi, 0.0, [
+            mcrt.Layer(d=0.3, scatterer=ray, tau=0.3, omega=1.0),
+            mcrt.Layer(d=1.5 * sp.kl, scatterer=sp, tau=1.5, omega=sp.omega)])),
+    ]
+    out = []
+    for i, (label, g) in enumerate(geos):
+        ct, st, w = midpoint_polar_grid(g, 90, 24)
+        cfg = mcrt.Config(g, number_of_rays=1024, maximum_scattering_times=10000, compute_cb=True,
+                          Ntheta=90, Nphi=24, costheta=ct, sintheta=st, inorm=w,
+                          thetab=np.deg2rad(BATCH_THETAB_DEG[::4]), Nphib=8,
+                          minimum_intensity=1e-6, tau0=50.0)
+        out.append(Request(cfg, g, I0, seed=9500 + i, precision="mixed" if i % 2 == 0 else "fp64",
+                           instance=200 + i, meta=dict(index=100 + i, label=label)))
+    corners = _instances()
+    return out + [corners[5], corners[10]]
+
+
+@pytest.mark.timeout(400)
+def test_batched_and_served_layered_and_spline_solves_match_the_oracle():
+    reqs = _layered_and_spline()
+    with Batch(reqs) as b:
+        b.launch()
+        b.wait()
+        for i, r in enumerate(reqs):
+            st = {}
+            _judge(r, b.read(i, stats=st), st, "batch")
+    with Server() as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        for t, r in zip(tickets, reqs):
+            st = {}
+            _judge(r, srv.result(t, stats=st), st, "served")
 
 
 @pytest.mark.timeout(400)
