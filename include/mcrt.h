@@ -372,6 +372,11 @@ typedef struct mcrt_request {
                            evaluations per chunk from the solve's bins and a
                            model of its events per ray, 64..8192 rays (a solve
                            is done no sooner than its longest chunk: latency). */
+  int32_t slots;        /* ray slots per workgroup while a chunk of this solve
+                           runs, and where its S table is read: slots and
+                           stage_s of mcrt_launch_shape (what the autotuner
+                           chooses; 0: the defaults) */
+  int32_t stage_s;
 } mcrt_request;
 
 typedef struct mcrt_batch mcrt_batch;

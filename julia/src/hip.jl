@@ -120,6 +120,8 @@ struct McrtRequest            # mcrt_request
     instance::UInt32
     precision::Int32
     chunk_rays::Int64
+    slots::Int32              # launch shape of the solve's chunks (0: the defaults)
+    stage_s::Int32
 end
 
 # ---------------------------------------------------------------- errors
@@ -322,7 +324,7 @@ function simulate_batch_hip(jobs; precision::Symbol = :mixed)
         reqs = [McrtRequest(Base.unsafe_convert(Ptr{McrtConfig}, cs[i]),
                             Base.unsafe_convert(Ptr{McrtGeometry}, gs[i]),
                             Tuple(Float64.(jobs[i][3])), UInt64(jobs[i][4]), 0, 0,
-                            PRECISION[precision], 0) for i in eachindex(jobs)]
+                            PRECISION[precision], 0, 0, 0) for i in eachindex(jobs)]
         b = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:mcrt_batch_create, libmcrt), Cint,
                     (Ptr{Cvoid}, Int32, Ptr{McrtRequest}, Ref{Ptr{Cvoid}}),

@@ -76,6 +76,8 @@ extern "C" int mcrt_batch_create(mcrt_ctx* ctx, int32_t n, const mcrt_request* r
     if (reqs[i].precision != MCRT_PRECISION_FP64 && reqs[i].precision != MCRT_PRECISION_MIXED)
       return fail(ctx, MCRT_ERR_UNSUPPORTED, "request %d: batches run the fp64 and mixed forms", i);
     if (reqs[i].chunk_rays < 0) return fail(ctx, MCRT_ERR_INVALID, "request %d: chunk_rays < 0", i);
+    if (reqs[i].slots < 0 || reqs[i].slots % 64)
+      return fail(ctx, MCRT_ERR_INVALID, "request %d: slots must be 0 or a positive multiple of 64", i);
   }
   HIP_OK(ctx, hipSetDevice(ctx->device));
   auto* B = new mcrt_batch();
@@ -87,7 +89,7 @@ extern "C" int mcrt_batch_create(mcrt_ctx* ctx, int32_t n, const mcrt_request* r
     int rc = mcrt::solve_create(B->solves[i], ctx->lds_per_block, q.cfg, q.geom, q.I0, q.seed,
                                 q.ray_begin, q.instance, q.precision == MCRT_PRECISION_MIXED,
                                 q.chunk_rays, mcrt::ChunkRule::Throughput, alloc, &ctx->err,
-                                ctx->tables);
+                                ctx->tables, q.slots, q.stage_s);
     if (rc) {
       std::string m = ctx->err;
       delete B;

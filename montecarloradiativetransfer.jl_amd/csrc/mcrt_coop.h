@@ -95,10 +95,12 @@ enum class ChunkRule { Throughput, Latency };
 double solve_work_per_ray(const mcrt_config* cfg, const mcrt_geometry* g);
 int64_t default_chunk_rays(int64_t N, double work_per_ray, ChunkRule rule);
 // Tables go to memory from alloc, or are shared through cache when given.
+// slots, stage_s: the launch shape of the solve's chunks (mcrt_launch_shape;
+// 0: the defaults).
 int solve_create(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, const mcrt_geometry* geom,
                  const double I0[4], uint64_t seed, uint64_t ray0, uint32_t instance, bool mixed,
                  int64_t chunk_rays, ChunkRule rule, const DevAlloc& alloc, std::string* err,
-                 TableCache* cache = nullptr);
+                 TableCache* cache = nullptr, int slots = 0, int stage_s = 0);
 // Device bytes of a solve apart from cached tables (for admission).
 size_t solve_bytes_estimate(const mcrt_config* cfg, const mcrt_geometry* g, bool mixed,
                             int64_t chunk_rays, ChunkRule rule);

@@ -2278,14 +2278,14 @@ template <typename R>
 int solve_create_impl(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, const mcrt_geometry* g,
                       const double I0[4], uint64_t seed, uint64_t ray0, uint32_t instance,
                       int64_t chunk_rays, ChunkRule rule, const DevAlloc& alloc, std::string* err,
-                      TableCache* cache) {
+                      TableCache* cache, int slots, int stage_s) {
   hipError_t e;
   sd.counters = (unsigned long long*)alloc(sizeof(unsigned long long) * kCntN, &e);
   if (!sd.counters) { *err = "batch: counter allocation failed"; return MCRT_ERR_NOMEM; }
   CoopParams<R> P;
   size_t lds = 0;
   int rc = init_params<R>(P, &lds, lds_max, cfg, g, I0, seed, ray0, instance, alloc, sd.counters, err,
-                          cache, &sd.table_refs);
+                          cache, &sd.table_refs, slots, stage_s >= 0);
   if (rc) return rc;
   // P.slots: the most slots the solve's LDS allows
   sd.mixed = sizeof(R) == 4;
@@ -2328,13 +2328,13 @@ int solve_create_impl(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, cons
 int solve_create(SolveDev& sd, size_t lds_max, const mcrt_config* cfg, const mcrt_geometry* geom,
                  const double I0[4], uint64_t seed, uint64_t ray0, uint32_t instance, bool mixed,
                  int64_t chunk_rays, ChunkRule rule, const DevAlloc& alloc, std::string* err,
-                 TableCache* cache) {
+                 TableCache* cache, int slots, int stage_s) {
   if (lds_max == 0) lds_max = 160 * 1024;
   lds_max -= 16;  // the batch kernel's static LDS (the chunk index)
   return mixed ? solve_create_impl<float>(sd, lds_max, cfg, geom, I0, seed, ray0, instance,
-                                          chunk_rays, rule, alloc, err, cache)
+                                          chunk_rays, rule, alloc, err, cache, slots, stage_s)
                : solve_create_impl<double>(sd, lds_max, cfg, geom, I0, seed, ray0, instance,
-                                           chunk_rays, rule, alloc, err, cache);
+                                           chunk_rays, rule, alloc, err, cache, slots, stage_s);
 }
 
 size_t solve_bytes_estimate(const mcrt_config* cfg, const mcrt_geometry* g, bool mixed,

@@ -345,7 +345,7 @@ int HipBackend::create(uint64_t job, int64_t* nchunks, double* cost, std::string
   const mcrt_request& q = t->r.req;
   int rc = mcrt::solve_create(t->sd, S_->ctx->lds_per_block, q.cfg, q.geom, q.I0, q.seed, q.ray_begin,
                               q.instance, t->mixed, q.chunk_rays, mcrt::ChunkRule::Latency, alloc, err,
-                              S_->cache);
+                              S_->cache, q.slots, q.stage_s);
   hipError_t e = hipSuccess;
   if (!rc) e = mcrt::solve_prepare(t->sd, setup_);
   if (!rc && e == hipSuccess) e = hipEventRecord(setup_ev_, setup_);
@@ -624,6 +624,8 @@ extern "C" int mcrt_submit(mcrt_server* S, const mcrt_request* q, uint64_t* tick
   if (q->precision != MCRT_PRECISION_FP64 && q->precision != MCRT_PRECISION_MIXED)
     return fail(nullptr, MCRT_ERR_UNSUPPORTED, "the server runs the fp64 and mixed forms");
   if (q->chunk_rays < 0) return fail(nullptr, MCRT_ERR_INVALID, "chunk_rays < 0");
+  if (q->slots < 0 || q->slots % 64)
+    return fail(nullptr, MCRT_ERR_INVALID, "slots must be 0 or a positive multiple of 64");
   auto* t = new Ticket();
   own(*q, t->r);
   t->mixed = q->precision == MCRT_PRECISION_MIXED;

@@ -142,6 +142,8 @@ class mcrt_request(C.Structure):
         ("instance", C.c_uint32),
         ("precision", C.c_int32),
         ("chunk_rays", C.c_int64),
+        ("slots", C.c_int32),
+        ("stage_s", C.c_int32),
     ]
 
 

@@ -37,6 +37,10 @@ class Request:
     ray_begin: int = 0
     instance: int = 0
     chunk_rays: int = 0
+    # launch shape of the solve's chunks: ray slots per workgroup and where the
+    # S table is read (mcrt_launch_shape; 0: the defaults, mcrt.tune.shape_for)
+    slots: int = 0
+    stage_s: int = 0
     meta: dict = field(default_factory=dict)
 
 
@@ -64,6 +68,8 @@ class Batch:
             q.instance = int(r.instance)
             q.precision = _abi.PRECISIONS[r.precision]
             q.chunk_rays = int(r.chunk_rays)
+            q.slots = int(r.slots)
+            q.stage_s = int(r.stage_s)
         h = C.c_void_p()
         _abi.check(self.ctx.lib.mcrt_batch_create(self.ctx.handle, len(reqs), arr, C.byref(h)),
                    self.ctx.handle)
@@ -195,6 +201,8 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--launches", type=int, default=2)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--tuned", action="store_true",
+                    help="launch shape of each solve kind from the autotuner (mcrt.tune.apply_tuned)")
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -206,6 +214,16 @@ def main(argv=None) -> int:
     for r in reqs:
         k = f"{r.meta['geometry']}/{r.meta['scatterer']}"
         kinds[k] = kinds.get(k, 0) + 1
+    tuned = None
+    if a.tuned:
+        from .tune import apply_tuned
+        t = time.perf_counter()
+        res = apply_tuned(reqs, log=log)
+        tuned = {"seconds": time.perf_counter() - t,
+                 "shapes": [{"kind": f"{k[0]}/{'+'.join(n for n, _ in k[1])}", "slots": v["slots"],
+                             "stage_s": v["stage_s"], "rays_per_s": v["rays_per_s"]}
+                            for k, v in res.items()]}
+        log(f"tuned: {tuned}")
     with Batch(reqs) as b:
         info = b.info()
         log(f"batch: {info}")
@@ -234,7 +252,7 @@ def main(argv=None) -> int:
                       "precision": a.precision, "kinds": kinds, "chunks": info["chunks"],
                       "device_bytes": info["device_bytes"], "wall_s": wall, "device_ms": dev,
                       "rays_per_s": rays / wall, "tflops_algorithmic": flops / (dev * 1e-3) / 1e12,
-                      **tot}), flush=True)
+                      "tuned": tuned, **tot}), flush=True)
     return 0
 
 

@@ -46,6 +46,8 @@ class Server:
         q.instance = int(r.instance)
         q.precision = _abi.PRECISIONS[r.precision]
         q.chunk_rays = int(r.chunk_rays)
+        q.slots = int(r.slots)
+        q.stage_s = int(r.stage_s)
         t = C.c_uint64()
         _abi.check(self.ctx.lib.mcrt_submit(self.handle, C.byref(q), C.byref(t)))
         with self._lock:

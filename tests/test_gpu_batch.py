@@ -128,8 +128,40 @@ def test_launch_shapes_and_autotuner():
         assert err <= 1e-5 * float(np.max(np.abs(ref)))  # same rays, FP32 grouping only
     res = autotune(cfg, g, I0, precision="mixed", sample_rays=32768, cache={})
     print(f"[autotune] {res['shape']} {res['rays_per_s']:.4g} rays/s", flush=True)
-    assert len(res["trials"]) == 8 and res["shape"] in [(t["slots"], t["workgroups_per_cu"])
-                                                         for t in res["trials"]]
+    # slots × workgroups per CU × S in LDS or HBM (mixed, CB on, Mie)
+    assert len(res["trials"]) == 16
+    assert {t["stage_s"] for t in res["trials"]} == {1, -1}
+    assert res["shape"] in [(t["slots"], t["workgroups_per_cu"], t["stage_s"]) for t in res["trials"]]
+
+
+def test_tuned_batch_shapes_are_applied_and_change_only_rounding():
+    """apply_tuned writes the tuned (slots, S placement) into the requests; a
+    batch run with them equals the default batch up to FP32 grouping (the same
+    rays, events and chunks), and the per-solve shape reaches the device (a
+    shape that cannot fit is refused)."""
+    from mcrt.tune import apply_tuned
+    reqs = _mix()
+    ref = []
+    with Batch(reqs) as b:
+        b.launch()
+        b.wait()
+        ref = [flat(b.read(i)) for i in range(len(reqs))]
+    tuned = [Request(r.cfg, r.g, r.I0, seed=r.seed, precision=r.precision) for r in reqs]
+    res = apply_tuned(tuned, sample_chunks=64, cache={})
+    print(f"[apply_tuned] {[(r.slots, r.stage_s) for r in tuned]}", flush=True)
+    assert all(r.slots in (64, 128, 192, 256) for r in tuned) and res
+    forced = [Request(r.cfg, r.g, r.I0, seed=r.seed, precision=r.precision, slots=64, stage_s=-1)
+              for r in reqs]
+    for rs in (tuned, forced):
+        with Batch(rs) as b:
+            b.launch()
+            b.wait()
+            for i, r in enumerate(rs):
+                st = {}
+                out = flat(b.read(i, stats=st))
+                err = float(np.max(np.abs(out - ref[i])))
+                tol = (1e-5 if r.precision == "mixed" else 1e-12) * float(np.max(np.abs(ref[i])))
+                assert err <= tol, (i, r.slots, r.stage_s, err)
 
 
 def _expected_chunks(N, tau, omega, bins, cb_bins, rule, min_int=1e-6, maxscat=10000):
