@@ -1086,7 +1086,11 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
       break;
     }
     {
-      Slot<R> s;
+      // value-initialised: a slot that starts the iteration empty gets every
+      // field it later reads from the refill, and no field of s is left for
+      // the optimiser to choose (uninitialised, the FP64 general-kind kernel
+      // was compiled to start a refilled ray's transport at RNG draw 0)
+      Slot<R> s{};
       if (live) s = *myslot;
       s.live = live;
       // ------------------------------------- refill free slots (in order)
@@ -1383,34 +1387,53 @@ __global__ __launch_bounds__(kThreads) void coop_kernel(const CoopParams<R>* __r
                                   slot_block<R>(p.slot_mem, blockIdx.x));
 }
 
+// A uniform pointer in scalar registers (readfirstlane of both halves), so
+// that a called function loads through it with scalar loads.
+template <typename T> MCRT_DEV T* uniform_ptr(T* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (T*)(((unsigned long long)hi << 32) | lo);
+}
+
+// One chunk of a batched launch, for one solve variant.  Not inlined: each
+// variant is register-allocated on its own (inlined into the one persistent
+// kernel, every variant's loops would pay for the register pressure of the
+// heaviest).  It reaches LDS through the kernel's dynamic LDS symbol, so its
+// LDS accesses stay LDS instructions.
+extern __shared__ __align__(16) unsigned char batch_lds[];
 template <typename R, int GEO, int SK, bool CB>
-MCRT_DEV void batch_chunk(const CoopParams<R>& p, bool fresh, const BatchChunk& ch,
-                          unsigned char* lds, Slot<R>* slots, const unsigned int* preempt,
-                          int64_t* state) {
-  const Lds<R> L = lds_layout<R, SK, CB>(p, lds);
-  if (fresh) {
+__device__ __attribute__((noinline)) void batch_chunk(const CoopParams<R>* pp_, int fresh,
+                                                      const BatchChunk* chp_, Slot<R>* slots_,
+                                                      const unsigned int* preempt_,
+                                                      int64_t* state_) {
+  const CoopParams<R>& p = *uniform_ptr(pp_);
+  const BatchChunk* chp = uniform_ptr(chp_);
+  Slot<R>* const slots = uniform_ptr(slots_);
+  const unsigned int* preempt = uniform_ptr(preempt_);
+  int64_t* const state = uniform_ptr(state_);
+  const Lds<R> L = lds_layout<R, SK, CB>(p, batch_lds);
+  if (__builtin_amdgcn_readfirstlane(fresh)) {
     stage_tables<R>(p, L, threadIdx.x);
     __syncthreads();
   }
-  run_rays<R, GEO, SK, CB, true>(p, L, ch.rb, ch.re, ch.slab, slots, preempt, ch.resume != 0,
-                                 state);
+  run_rays<R, GEO, SK, CB, true>(p, L, chp->rb, chp->re, chp->slab, slots, preempt,
+                                 chp->resume != 0, state);
 }
 
 template <typename R, int GEO, int SK>
-MCRT_DEV void batch_chunk_cb(const CoopParams<R>& p, bool fresh, const BatchChunk& ch,
-                             unsigned char* lds, Slot<R>* slots, const unsigned int* preempt,
-                             int64_t* state) {
-  if (p.compute_cb) batch_chunk<R, GEO, SK, true>(p, fresh, ch, lds, slots, preempt, state);
-  else batch_chunk<R, GEO, SK, false>(p, fresh, ch, lds, slots, preempt, state);
+MCRT_DEV void batch_chunk_cb(const CoopParams<R>* pp, int fresh, const BatchChunk* chp,
+                             Slot<R>* slots, const unsigned int* preempt, int64_t* state) {
+  if (pp->compute_cb) batch_chunk<R, GEO, SK, true>(pp, fresh, chp, slots, preempt, state);
+  else batch_chunk<R, GEO, SK, false>(pp, fresh, chp, slots, preempt, state);
 }
 
 template <typename R, int GEO>
-MCRT_DEV void batch_chunk_geo(const CoopParams<R>& p, bool fresh, const BatchChunk& ch,
-                              unsigned char* lds, Slot<R>* slots, const unsigned int* preempt,
-                              int64_t* state) {
-  if (p.sk == kMie) batch_chunk_cb<R, GEO, kMie>(p, fresh, ch, lds, slots, preempt, state);
-  else if (p.sk == kRay) batch_chunk_cb<R, GEO, kRay>(p, fresh, ch, lds, slots, preempt, state);
-  else batch_chunk_cb<R, GEO, kAny>(p, fresh, ch, lds, slots, preempt, state);
+MCRT_DEV void batch_chunk_geo(const CoopParams<R>* pp, int fresh, const BatchChunk* chp,
+                              Slot<R>* slots, const unsigned int* preempt, int64_t* state) {
+  if (pp->sk == kMie) batch_chunk_cb<R, GEO, kMie>(pp, fresh, chp, slots, preempt, state);
+  else if (pp->sk == kRay) batch_chunk_cb<R, GEO, kRay>(pp, fresh, chp, slots, preempt, state);
+  else batch_chunk_cb<R, GEO, kAny>(pp, fresh, chp, slots, preempt, state);
 }
 
 // Many solves in one persistent launch: each workgroup takes the next chunk
@@ -1426,7 +1449,6 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(const BatchChunk* __res
                                                          int nchunks, unsigned int* __restrict__ next,
                                                          void* slot_mem, int64_t* __restrict__ states,
                                                          const unsigned int* preempt) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ int chunk_id;
   Slot<R>* const own = slot_block<R>(slot_mem, blockIdx.x);
   const void* cur = nullptr;
@@ -1440,14 +1462,15 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(const BatchChunk* __res
     const int c = chunk_id;
     __syncthreads();
     if (c >= nchunks) break;
-    const BatchChunk ch = chunks[c];
-    const CoopParams<R>& p = *(const CoopParams<R>*)ch.params;
-    const bool fresh = ch.params != cur;
-    cur = ch.params;
-    Slot<R>* const slots = ch.cont ? (Slot<R>*)ch.cont : own;
+    const BatchChunk* const chp = chunks + c;
+    const void* const params = chp->params;
+    const CoopParams<R>* const pp = (const CoopParams<R>*)params;
+    const int fresh = params != cur;
+    cur = params;
+    Slot<R>* const slots = chp->cont ? (Slot<R>*)chp->cont : own;
     int64_t* const st = states ? states + c : nullptr;
-    if (p.geo_kind == 0) batch_chunk_geo<R, 0>(p, fresh, ch, lds_raw, slots, preempt, st);
-    else batch_chunk_geo<R, 1>(p, fresh, ch, lds_raw, slots, preempt, st);
+    if (pp->geo_kind == 0) batch_chunk_geo<R, 0>(pp, fresh, chp, slots, preempt, st);
+    else batch_chunk_geo<R, 1>(pp, fresh, chp, slots, preempt, st);
   }
 }
 
