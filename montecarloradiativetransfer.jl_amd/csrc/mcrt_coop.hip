@@ -75,6 +75,10 @@ template <typename R> struct Shape;
 template <> struct Shape<float> {
   static constexpr int kSlots = 256;
   static constexpr int kChunk = 8;
+  // peel-off chunk of the batched kernel: its chunks are mostly short ray
+  // ranges whose iterations are few and tail-heavy, where the 8/9-bin passes'
+  // register footprint costs more than their per-event savings
+  static constexpr int kChunkBatch = 4;
   static constexpr bool kStageS = true;  // S table in LDS
   // ρ² below which a bin counts as aligned with the event direction (the
   // polar branch): FP32 rounding leaves ρ² ≈ 1e-13 for exactly (anti)parallel
@@ -84,6 +88,7 @@ template <> struct Shape<float> {
 template <> struct Shape<double> {
   static constexpr int kSlots = 128;
   static constexpr int kChunk = 2;
+  static constexpr int kChunkBatch = 2;
   static constexpr bool kStageS = false;
   static constexpr double kPolar2 = 1e-24;  // the reference's x² + y² < 1e-24
 };
@@ -615,11 +620,10 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const TabC<R>& t, const R* F, cons
 }
 
 // One θ-range of the peel-off with the table set (t, F): full register chunks
-// of Shape<R>::kChunk bins, then the remainder in passes of 4, 2 and 1.
-template <typename R, int GEO, int SK, bool CUT>
+// of CH bins, then the remainder in passes of 4, 2 and 1.
+template <int CH, typename R, int GEO, int SK, bool CUT>
 MCRT_DEV void rt_range(const CoopParams<R>& p, const TabC<R>& t, const R* F, const R* rtrec, R* esc,
                        double* slab, int E, int tid, int lane, int wave, int rg) {
-  constexpr int CH = Shape<R>::kChunk;
   const int jb = p.range_b[rg], je = p.range_e[rg];
   const int nrow = je - jb;
   const int nb = nrow * p.Np;  // bins of this range
@@ -659,23 +663,23 @@ MCRT_DEV void rt_range(const CoopParams<R>& p, const TabC<R>& t, const R* F, con
 // only range) uses table set 0 (F0), the plane's lower range table set 1 (F1).
 // kAny dispatches on the set's kind once per range: a sphere's set is a
 // Spline (Rayleigh and Mie spheres have kernels of their own), a plane's any.
-template <typename R, int GEO, int SK, bool CUT>
+template <int CH, typename R, int GEO, int SK, bool CUT>
 MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F0, const R* F1, const R* rtrec, R* esc,
                         double* slab, int E, int tid, int lane, int wave) {
   for (int rg = 0; rg < p.nranges; ++rg) {
     if constexpr (SK != kAny) {
-      rt_range<R, GEO, SK, CUT>(p, p.tc[0], F0, rtrec, esc, slab, E, tid, lane, wave, rg);
+      rt_range<CH, R, GEO, SK, CUT>(p, p.tc[0], F0, rtrec, esc, slab, E, tid, lane, wave, rg);
     } else {
       const int set = p.range_kind[rg] == 1 ? 0 : 1;
       const int kind = set ? p.sca_lo.kind : p.sca.kind;
       const R* F = set ? F1 : F0;
       if (kind == kSpl) {
-        rt_range<R, GEO, kSpl, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);
+        rt_range<CH, R, GEO, kSpl, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);
       } else if constexpr (GEO == 0) {
         if (kind == kMie)
-          rt_range<R, GEO, kMie, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);
+          rt_range<CH, R, GEO, kMie, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);
         else
-          rt_range<R, GEO, kRay, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);
+          rt_range<CH, R, GEO, kRay, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);
       }
     }
   }
@@ -1298,10 +1302,11 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
     }
 
     // ------------------------------- peel-off: bins × events (cpu.jl:6-31)
+    constexpr int RCH = ATOMIC ? Shape<R>::kChunkBatch : Shape<R>::kChunk;
     if (GEO == 1 && !p.rt_cut)
-      rt_ranges<R, GEO, SK, false>(p, L.F, F1, rtrec, esc, slab, E, tid, lane, wave);
+      rt_ranges<RCH, R, GEO, SK, false>(p, L.F, F1, rtrec, esc, slab, E, tid, lane, wave);
     else
-      rt_ranges<R, GEO, SK, true>(p, L.F, F1, rtrec, esc, slab, E, tid, lane, wave);
+      rt_ranges<RCH, R, GEO, SK, true>(p, L.F, F1, rtrec, esc, slab, E, tid, lane, wave);
 
     // ------------------------------------------ CB bins × events (cpu.jl:38-160)
     if (CB && p.compute_cb) cb_all<R, GEO, SK, ATOMIC>(p, L.F, Sbins, cbrec, cbdx, slab, E, tid);
