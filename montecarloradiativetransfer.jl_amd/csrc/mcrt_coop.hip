@@ -485,16 +485,21 @@ MCRT_DEV void lmap_normalise(Lmap<double>& Q, double& lq) {
   }
 }
 
-// Per-slot ray state (HBM between phases).
-template <typename R> struct Slot {
+// Per-slot ray state (HBM between phases): the transport part, copied into
+// registers by the transport and update phases, and the CB path state, read
+// and written in place by cb_event alone (a whole-record copy kept ~300 B per
+// lane in scratch memory across the transport).
+template <typename R> struct RaySt {
   int live;
-  int idx, idxp, idx1;
+  int idx, idxp;
   int64_t n;
   uint64_t ray;
   uint32_t draw;
   R I[4], Ip[4];
   V3<R> k, kp, x, xp;
-  // CB path state (FP64)
+};
+struct CbSt {  // FP64
+  int idx1;         // layer of the first scattering point
   V3<double> x1d;   // first scattering point
   V3<double> kprev; // previous incoming direction
   Fr<double> ksf;   // frame of the last forward step
@@ -504,6 +509,7 @@ template <typename R> struct Slot {
   double lq;
   Fr<double> kbs;   // frame of -k₂
 };
+template <typename R> struct Slot : RaySt<R>, CbSt {};
 
 // Per-slot running sums of the energy budget and the work counters, kept in
 // HBM beside the slot state (not in registers across the bin loops).
@@ -1094,7 +1100,7 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
       // field it later reads from the refill, and no field of s is left for
       // the optimiser to choose (uninitialised, the FP64 general-kind kernel
       // was compiled to start a refilled ray's transport at RNG draw 0)
-      Slot<R> s{};
+      RaySt<R> s{};
       if (live) s = *myslot;
       s.live = live;
       // ------------------------------------- refill free slots (in order)
@@ -1169,11 +1175,12 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
             s.live = 0;
           }
           if (CB) {
-            s.Ea1 = {p.Ei[0], p.Ei[1]};
-            s.Ea2 = {p.Ei[2], p.Ei[3]};
-            s.enorma = 0.0;
-            s.ksf = {p.f0[0], p.f0[1], p.f0[2], p.f0[3]};
-            s.lq = 0.0;
+            CbSt& cs = *myslot;
+            cs.Ea1 = {p.Ei[0], p.Ei[1]};
+            cs.Ea2 = {p.Ei[2], p.Ei[3]};
+            cs.enorma = 0.0;
+            cs.ksf = {p.f0[0], p.f0[1], p.f0[2], p.f0[3]};
+            cs.lq = 0.0;
           }
         }
         next_ray += tot;
@@ -1285,7 +1292,7 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
         }
       }
       live = s.live;
-      if (owner && live) *myslot = s;
+      if (owner && live) static_cast<RaySt<R>&>(*myslot) = s;
       if (tid == 0) wcnt[31] = E;
     }
     __syncthreads();
@@ -1317,7 +1324,7 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
 
     // ---------------------- weight update and termination (cpu.jl:263-272)
     if (owner && live) {
-      Slot<R> s = *myslot;
+      RaySt<R> s = *myslot;
       R Iref = R(0), Itrans = R(0);
       for (int w = 0; w < kWaves; ++w) {
         Iref += esc[(e_me * kWaves + w) * 2];
@@ -1344,7 +1351,7 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
         Aacc[4] += (double)s.I[0];
         live = 0;
       }
-      if (live) *myslot = s;
+      if (live) static_cast<RaySt<R>&>(*myslot) = s;
     }
     __syncthreads();
   }
