@@ -639,8 +639,10 @@ MCRT_DEV void rt_range(const CoopParams<R>& p, const TabC<R>& t, const R* F, con
   int c0 = 0;
   // with chunks of 8, a remainder of one bin joins the last chunk (a pass of
   // 9: e.g. 17 bins per thread at 180 × 48 run as 9 + 8, and 9 as one pass),
-  // since every pass pays the per-event overhead (record reads, reduction)
-  if constexpr (CH == 8) {
+  // since every pass pays the per-event overhead (record reads, reduction).
+  // Not for a plane's tabulated scatterers: their 9-bin pass spills ~30
+  // registers per event around the polar block (8 + 1 is 8% faster there)
+  if constexpr (CH == 8 && (GEO == 1 || SK == kRay)) {
     if (per % 8 == 1 && per >= 9) {
       rt_pass<9, R, GEO, SK, CUT>(p, t, F, rtrec, esc, slab, E, tid, lane, wave, jb, nrow, nb, c0,
                                   fw, slot2);
@@ -680,7 +682,10 @@ MCRT_DEV void rt_ranges(const CoopParams<R>& p, const R* F0, const R* F1, const 
       const int kind = set ? p.sca_lo.kind : p.sca.kind;
       const R* F = set ? F1 : F0;
       if (kind == kSpl) {
-        rt_range<CH, R, GEO, kSpl, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);
+        // a plane's Spline pass of 8 bins spills ~40 registers per event: 4
+        // bins per pass there (1-2% faster)
+        constexpr int SCH = (GEO == 0 && CH > 4) ? 4 : CH;
+        rt_range<SCH, R, GEO, kSpl, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);
       } else if constexpr (GEO == 0) {
         if (kind == kMie)
           rt_range<CH, R, GEO, kMie, CUT>(p, p.tc[set], F, rtrec, esc, slab, E, tid, lane, wave, rg);

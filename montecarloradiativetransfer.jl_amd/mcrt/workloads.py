@@ -14,6 +14,8 @@
                         scatterer the Mie form cannot express)
   plane_rayleigh_over_mie  two layers: Rayleigh τ = 0.5 over the C2 Mie
                         scatterer τ = 1.5, normal incidence, CB on
+  plane_mie_cb          plane slab τ = 1 of the C2 Mie scatterer, normal incidence,
+                        sphere_mie_cb's RT and CB bins, 1e6 rays, seed 13
 The ray counts of the full workloads are 1e6, 1e7 and 1e9; tests use fewer rays
 of the same configuration.
 """
@@ -117,6 +119,20 @@ def plane_rayleigh_over_mie(n_rays: int = 10**6, *, tau_top: float = 0.5, tau_mi
     return cfg, g, 11
 
 
+def plane_mie_cb(n_rays: int = 10**6, *, tau: float = 1.0, compute_cb: bool = True):
+    """A plane slab of the C2 Mie scatterer at normal incidence with CB on the
+    sphere workload's bins (RT 180 × 48 midpoint bins, θb 0:0.05°:5° × 8)."""
+    mie = c2_mie(1801)
+    g = PlaneGeometry(math.pi, 0.0, [Layer(d=tau * mie.kl, scatterer=mie, tau=tau, omega=mie.omega)])
+    ct, st, w = midpoint_polar_grid(g, 180, 48)
+    cfg = Config(g, number_of_rays=n_rays, maximum_scattering_times=10000, compute_cb=compute_cb,
+                 Ntheta=180, Nphi=48, costheta=ct, sintheta=st, inorm=w,
+                 thetab=np.deg2rad(np.arange(101) * 0.05), Nphib=8, minimum_intensity=1e-6,
+                 tau0=50.0)
+    return cfg, g, 13
+
+
 WORKLOADS = {"plane_rayleigh_tau1": plane_rayleigh_tau1, "sphere_mie_cb": sphere_mie_cb,
              "plane_rayleigh_cb": plane_rayleigh_cb, "sphere_mie_rt": sphere_mie_rt,
-             "sphere_spline_cb": sphere_spline_cb, "plane_rayleigh_over_mie": plane_rayleigh_over_mie}
+             "sphere_spline_cb": sphere_spline_cb, "plane_rayleigh_over_mie": plane_rayleigh_over_mie,
+             "plane_mie_cb": plane_mie_cb}
