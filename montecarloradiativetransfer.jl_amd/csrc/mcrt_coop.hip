@@ -198,6 +198,83 @@ MCRT_DEV void kahan(double* s, double* c, double x) {
   *s = t;
 }
 
+// FP64 building blocks of the FP64 form's bin loops.  The library's are
+// correctly rounded over their whole domains, with range and special-case
+// handling (on gfx950: acos 93 VALU instructions, exp 42, sqrt 22, a division
+// 12); these are branch-free, within a few ulp on the arguments the loops
+// pass, and cost 17-25, 20, 10 and 5.
+namespace f64 {
+// 1/x for a normal, finite x ≠ 0: v_rcp_f64 and two Newton steps.
+MCRT_DEV double rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-x, y, 1.0);
+  return __builtin_fma(y, e, y);
+}
+// √x for x ≥ 0: v_rsq_f64, one coupled Newton step on (√x, 1/(2√x)) and a
+// final correction of √x by its residual; 0 below 2^-1000 (where the rsq of a
+// flushed argument would give 0·∞).
+MCRT_DEV double sqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double e = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  const double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return x > 0x1p-1000 ? g : 0.0;
+}
+// e^{-t}: -t = n·ln2 + r with n an integer and |r| ≤ ln2/2 (Cody and Waite's
+// reduction, ln2 split at 32 bits so that n·ln2_hi is exact), e^r by its Taylor
+// polynomial of degree 13 (remainder < 5e-18), times 2^n.  NaN stays NaN; the
+// clamps keep 2^n within v_ldexp_f64's range (results below 2^-1075 are 0).
+MCRT_DEV double exp_neg(double t) {
+  double x = -t;
+  x = x < -746.0 ? -746.0 : (x > 709.0 ? 709.0 : x);
+  const double n = __builtin_rint(x * 1.4426950408889634);
+  double r = __builtin_fma(n, -0.6931471803691238, x);  // ln2_hi = 0x1.62e42feep-1
+  r = __builtin_fma(n, -1.9082149292705877e-10, r);    // ln2 - ln2_hi
+  double p = 1.0 / 6227020800.0;                       // 1/13!
+  p = __builtin_fma(p, r, 1.0 / 479001600.0);
+  p = __builtin_fma(p, r, 1.0 / 39916800.0);
+  p = __builtin_fma(p, r, 1.0 / 3628800.0);
+  p = __builtin_fma(p, r, 1.0 / 362880.0);
+  p = __builtin_fma(p, r, 1.0 / 40320.0);
+  p = __builtin_fma(p, r, 1.0 / 5040.0);
+  p = __builtin_fma(p, r, 1.0 / 720.0);
+  p = __builtin_fma(p, r, 1.0 / 120.0);
+  p = __builtin_fma(p, r, 1.0 / 24.0);
+  p = __builtin_fma(p, r, 1.0 / 6.0);
+  p = __builtin_fma(p, r, 0.5);
+  p = __builtin_fma(p, r, 1.0);
+  p = __builtin_fma(p, r, 1.0);
+  return __builtin_amdgcn_ldexp(p, (int)n);
+}
+// acos(a) for a in [0, 1]: √(1-a)·p(a) with p the degree-17 weighted minimax
+// polynomial of csrc/fit_acos.py (|acos error| ≤ 8.5e-17 rad before rounding),
+// highest degree first.
+constexpr int kAcosTerms64 = 18;
+MCRT_DEV double acos01(double a) {
+  constexpr double c[kAcosTerms64] = {
+      -9.2665172311424016e-6, 9.0838956934216434e-5, -0.00042183485744450771,
+      0.0012463190146607077, -0.0026692207269505534, 0.0045095056300445343,
+      -0.0064400495955701362, 0.0082756854246340757, -0.010099913020109592,
+      0.012182022890173891, -0.014879123878712378, 0.018671238874509465,
+      -0.024374217369112181, 0.033681279022635642, -0.050792814208705652,
+      0.089048622540930108, -0.21460183660249109, 1.5707963267948965};
+  double p = c[0];
+#pragma unroll
+  for (int k = 1; k < kAcosTerms64; ++k) p = __builtin_fma(p, a, c[k]);
+  return sqrt(1.0 - a) * p;
+}
+}  // namespace f64
+// 1/x and √x of the bin loops (the FP32 hardware instructions; FP64: f64::).
+MCRT_DEV float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+MCRT_DEV double recip(double x) { return f64::rcp(x); }
+MCRT_DEV float bsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+MCRT_DEV double bsqrt(double x) { return f64::sqrt(x); }
+
 // Table coordinate sn = acos(μ)·(Nθ-1)/π of a phase / amplitude lookup
 // (mie.jl:53-55).  FP32: acos(a) = √(1-a)·p(a) on [0, 1] with p the degree-6
 // weighted minimax polynomial of csrc/fit_acos.py (Remez exchange, |acos error|
@@ -210,7 +287,8 @@ MCRT_DEV void kahan(double* s, double* c, double x) {
 // forms 1 - a from μ.
 template <typename R> MCRT_DEV R theta_index(const TabC<R>& t, R mu, R s2 = R(-1));
 template <> MCRT_DEV double theta_index(const TabC<double>& t, double mu, double) {
-  return ::acos(clamp1(mu)) * t.scale;
+  const double r = f64::acos01(__builtin_fabs(clamp1(mu))) * t.scale;
+  return mu < 0.0 ? t.pis - r : r;
 }
 template <> MCRT_DEV float theta_index(const TabC<float>& t, float mu, float s2) {
   const float a = __builtin_fabsf(mu);
@@ -394,7 +472,7 @@ MCRT_DEV Plane<R> scat_plane(const R* m, R kx, R ky, R kz, R e1, R e2, R g0, R g
   Plane<R> q;
   R r2 = plane_raw(q, m, kx, ky, kz, e1, e2, g0, g1, g2);
   if (r2 < Shape<R>::kPolar2) r2 = plane_polar(q, m, e1, e2);
-  q.inv = M<R>::div(R(1), r2);
+  q.inv = recip(r2);
   return q;
 }
 
@@ -449,16 +527,45 @@ template <> struct ExpK<float> {
 };
 template <> struct ExpK<double> {
   static constexpr double k = 1.0;
-  static MCRT_DEV double exp_neg(double t) { return ::exp(-t); }
+  static MCRT_DEV double exp_neg(double t) { return f64::exp_neg(t); }
 };
 
-// cos/sin of the large FP64 CB phase, given in the unit PhaseUnit<R> (radians
-// for FP64; revolutions for FP32, so that an exact FP64 reduction to
-// [-1/2, 1/2] revolutions precedes the FP32 evaluation).
-template <typename R> struct PhaseUnit;
-template <> struct PhaseUnit<double> { static constexpr double per_radian = 1.0; };
-template <> struct PhaseUnit<float> { static constexpr double per_radian = 0.15915494309189535; };
-MCRT_DEV void phase_cs(double phi, double& c, double& s) { sincos(phi, &s, &c); }
+// cos/sin of the large FP64 CB phase, given in revolutions (kl·(k_i + k_f)
+// carries the factor 1/2π), so that its reduction to [-1/2, 1/2] revolutions,
+// rev - rint(rev), is exact.  FP32 evaluates the reduced phase with
+// v_sin_f32/v_cos_f32.  FP64 splits it further into quarter turns q and
+// h ∈ [-1/2, 1/2] (4·f - rint(4·f), exact) and evaluates sin(πh/2) and
+// cos(πh/2) by their Taylor polynomials (through h¹⁵ and h¹⁶: remainders
+// below 7e-17), rotated by q quarter turns; the library sincos of the phase in
+// radians costs 154 VALU instructions with a branch for large arguments.
+constexpr double kRevPerRadian = 0.15915494309189535;  // 1/2π
+MCRT_DEV void phase_cs(double rev, double& c, double& s) {
+  const double g = 4.0 * (rev - __builtin_rint(rev));
+  const double q = __builtin_rint(g);
+  const double h = g - q, z = h * h;
+  // (π/2)^(2k+1)/(2k+1)! and (π/2)^(2k)/(2k)! with alternating signs
+  double ps = -6.688035109811468e-10;
+  ps = __builtin_fma(ps, z, 5.692172921967927e-08);
+  ps = __builtin_fma(ps, z, -3.598843235212085e-06);
+  ps = __builtin_fma(ps, z, 0.00016044118478735983);
+  ps = __builtin_fma(ps, z, -0.004681754135318688);
+  ps = __builtin_fma(ps, z, 0.07969262624616705);
+  ps = __builtin_fma(ps, z, -0.6459640975062463);
+  ps = __builtin_fma(ps, z, 1.5707963267948966);
+  const double sn = h * ps;
+  double pc = 6.565963114979473e-11;
+  pc = __builtin_fma(pc, z, -6.386603083791852e-09);
+  pc = __builtin_fma(pc, z, 4.710874778818172e-07);
+  pc = __builtin_fma(pc, z, -2.5202042373060607e-05);
+  pc = __builtin_fma(pc, z, 0.0009192602748394266);
+  pc = __builtin_fma(pc, z, -0.02086348076335296);
+  pc = __builtin_fma(pc, z, 0.25366950790104803);
+  pc = __builtin_fma(pc, z, -1.2337005501361697);
+  pc = __builtin_fma(pc, z, 1.0);
+  const int qi = (int)q & 3;  // q ∈ {-2..2}: quarter turns mod 4
+  s = qi == 0 ? sn : (qi == 1 ? pc : (qi == 2 ? -sn : -pc));
+  c = qi == 0 ? pc : (qi == 1 ? -sn : (qi == 2 ? -pc : sn));
+}
 MCRT_DEV void phase_cs(double rev, float& c, float& s) {
   float f = (float)(rev - rint(rev));      // in [-1/2, 1/2] revolutions
   s = __builtin_amdgcn_sinf(f);            // v_sin_f32 takes revolutions
@@ -588,7 +695,7 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const TabC<R>& t, const R* F, cons
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
         Plane<R>& q = qs[i];
-        q.inv = M<R>::div(R(1), r2[i]);
+        q.inv = recip(r2[i]);
         R o[4];
         peel_I<R, SK>(t, F, rec + 8, rec + 12, q, o);
         R t;  // K·distance_to_boundary
@@ -596,7 +703,7 @@ MCRT_DEV void rt_pass(const CoopParams<R>& p, const TabC<R>& t, const R* F, cons
           t = (bk[i][2] > R(0) ? rec[16] : rec[16] + rec[17]) * -bk[i][9];
         } else {         // sphere.jl:69-73
           const R kx = bk[i][0] * rec[14] + bk[i][1] * rec[15] + bk[i][2] * rec[16];
-          t = M<R>::sqrt(kx * kx + rec[17]) - kx;  // rec[17] ≥ 0: the radicand is too
+          t = bsqrt(kx * kx + rec[17]) - kx;  // rec[17] ≥ 0: the radicand is too
         }
         // the τ₀ cut-off (cpu.jl:20); CUT = false when no distance can exceed it
         const R w = (!CUT || t <= tau0k) ? bk[i][8] * ExpK<R>::exp_neg(t) : R(0);
@@ -748,7 +855,7 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
       if (GEO == 0) ta = kfz > R(0) ? -xa.z * ikz : (xa.z + cr[30]) * -ikz;
       else {
         R kx = kfx * xa.x + kfy * xa.y + kfz * xa.z;
-        ta = M<R>::sqrt(kx * kx + cr[30]) - kx;
+        ta = bsqrt(kx * kx + cr[30]) - kx;
       }
       if (CUT && !(ta <= tau0k)) continue;
       const Plane<R> qa = scat_plane(cr + 8, kfx, kfy, kfz, b[3], b[4], b[5], b[6], b[7]);
@@ -770,7 +877,7 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
         if (GEO == 0) tb = kfz > R(0) ? -xb.z * ikz : (xb.z + cr[31]) * -ikz;
         else {
           R kx = kfx * xb.x + kfy * xb.y + kfz * xb.z;
-          tb = M<R>::sqrt(kx * kx + cr[31]) - kx;
+          tb = bsqrt(kx * kx + cr[31]) - kx;
         }
         const bool bok = !CUT || M<R>::fabs(tb + tab) <= tau0k;
         // the attenuations e^{-τ/2} and the 1/ρ² of each plane ride on peel_E's ψ rotation
@@ -790,7 +897,7 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
         Cx<R> e1 = a1 * ep + b1, e2 = a2 * ep + b2;
         I_from_E(e1, e2, o);
         // Ia/(Ia + Ib) (cpu.jl:114); 0 where both fields vanish (0·finite)
-        s0 = M<R>::div(Ia0, fmax_r(Ia0 + Ib0, std::numeric_limits<R>::min()));
+        s0 = Ia0 * recip(fmax_r(Ia0 + Ib0, std::numeric_limits<R>::min()));
         s12 = s0;
       }
       // the final rotation (cpu.jl:118-124) is linear, so it acts before the
@@ -2108,7 +2215,7 @@ int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_co
       b[9] = (R)(2.0 * cps * sps);
       b[10] = (R)(1.0 / kf[2]);
       // kl·(k_i + k_f) in the unit of phase_cs (cpu.jl:94)
-      const double ku = gd.kl1 * PhaseUnit<R>::per_radian;
+      const double ku = gd.kl1 * kRevPerRadian;
       kik[4 * lb] = ku * (k0[0] + kf[0]); kik[4 * lb + 1] = ku * (k0[1] + kf[1]);
       kik[4 * lb + 2] = ku * (k0[2] + kf[2]);
       kik[4 * lb + 3] = 0;

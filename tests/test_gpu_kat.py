@@ -15,8 +15,8 @@ Bounds: 1e-12 relative to the output's scale in the FP64 forms (fp64_plain
 repeats the reference's operations in order; fp64 evaluates the peel-off in
 the scattering-plane form, so it agrees to rounding).  The mixed form computes
 in FP32: its bound is FP32_REL (1e-5) of the output's scale, where the scale is
-max(1, max |reference|) — the FP32 arccos of the table lookup (|error| ≤ 2e-8
-rad, Abramowitz & Stegun 4.4.46) and the FP32 rounding of μ (≈ 6e-8/sinθ rad,
+max(1, max |reference|) — the FP32 arccos of the table lookup (|error| ≤ 8.8e-8
+rad, the degree-6 minimax fit of csrc/fit_acos.py) and the FP32 rounding of μ (≈ 6e-8/sinθ rad,
 ≲ 2.5e-4 of a 0.1° table step for |μ| ≤ 0.99) dominate; distances are bounded
 at FP32_REL of the larger of the medium's size and the distance.  Inputs keep |μ| ≤ 0.99 for Mie lookups,
 where the lookup is well conditioned; the polar branch (μ = ±1 exactly) is

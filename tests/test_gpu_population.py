@@ -89,7 +89,7 @@ def _oracle(r: Request):
     return _REF[key]
 
 
-def _judge(r: Request, out, st, where):
+def _check_vs_oracle(r: Request, out, st, where):
     ref = _oracle(r)
     form = r.precision
     label = f"{where}/{r.meta['index']}/{form}/{r.meta['label']}"
@@ -115,7 +115,7 @@ def test_batched_population_corners_match_the_oracle():
         print(f"batch {b.info()} {b.kernel_ms():.1f} ms", flush=True)
         for i, r in enumerate(reqs):
             st = {}
-            _judge(r, b.read(i, stats=st), st, "batch")
+            _check_vs_oracle(r, b.read(i, stats=st), st, "batch")
 
 
 def _layered_and_spline():
@@ -156,12 +156,12 @@ def test_batched_and_served_layered_and_spline_solves_match_the_oracle():
         b.wait()
         for i, r in enumerate(reqs):
             st = {}
-            _judge(r, b.read(i, stats=st), st, "batch")
+            _check_vs_oracle(r, b.read(i, stats=st), st, "batch")
     with Server() as srv:
         tickets = [srv.submit(r) for r in reqs]
         for t, r in zip(tickets, reqs):
             st = {}
-            _judge(r, srv.result(t, stats=st), st, "served")
+            _check_vs_oracle(r, srv.result(t, stats=st), st, "served")
 
 
 @pytest.mark.timeout(400)
@@ -177,4 +177,4 @@ def test_served_population_corners_match_the_oracle():
     print(f"server {s}", flush=True)
     assert s["completed"] == len(reqs) and s["failed"] == 0
     for r, out, st in outs:
-        _judge(r, out, st, "served")
+        _check_vs_oracle(r, out, st, "served")
