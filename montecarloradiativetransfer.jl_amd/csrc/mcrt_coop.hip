@@ -279,7 +279,8 @@ MCRT_DEV double bsqrt(double x) { return f64::sqrt(x); }
 // (mie.jl:53-55).  FP32: acos(a) = √(1-a)·p(a) on [0, 1] with p the degree-6
 // weighted minimax polynomial of csrc/fit_acos.py (Remez exchange, |acos error|
 // ≤ 8.8e-8 rad, below the FP32 rounding of θ and of μ itself), its coefficients
-// pre-multiplied by (Nθ-1)/π, and acos(-a) = π - acos(a).  FP64: the library acos.
+// pre-multiplied by (Nθ-1)/π, and acos(-a) = π - acos(a).  FP64: f64::acos01,
+// the same form at degree 17.
 // s2 ≥ 0 gives ρ² = 1 - μ² of the scattering plane: then 1 - a is formed as
 // ρ²/(1 + a), which keeps θ accurate within 0.02° of 0 and π where μ itself
 // rounds to ±1 in FP32 (the CB bins near exact backscattering, where the
