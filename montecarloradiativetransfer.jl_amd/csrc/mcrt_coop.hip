@@ -75,10 +75,10 @@ template <typename R> struct Shape;
 template <> struct Shape<float> {
   static constexpr int kSlots = 256;
   static constexpr int kChunk = 8;
-  // peel-off chunk of the batched kernel: its chunks are mostly short ray
-  // ranges whose iterations are few and tail-heavy, where the 8/9-bin passes'
-  // register footprint costs more than their per-event savings
-  static constexpr int kChunkBatch = 4;
+  // peel-off chunk of the batch kernel's plane variants with tabulated
+  // scatterers (Mie, general kind): at 8 bins their event loops spill 10-17
+  // registers per event
+  static constexpr int kChunkBatchPlane = 4;
   static constexpr bool kStageS = true;  // S table in LDS
   // ρ² below which a bin counts as aligned with the event direction (the
   // polar branch): FP32 rounding leaves ρ² ≈ 1e-13 for exactly (anti)parallel
@@ -88,7 +88,7 @@ template <> struct Shape<float> {
 template <> struct Shape<double> {
   static constexpr int kSlots = 128;
   static constexpr int kChunk = 2;
-  static constexpr int kChunkBatch = 2;
+  static constexpr int kChunkBatchPlane = 2;
   static constexpr bool kStageS = false;
   static constexpr double kPolar2 = 1e-24;  // the reference's x² + y² < 1e-24
 };
@@ -1422,7 +1422,8 @@ MCRT_DEV void run_rays(const CoopParams<R>& p, const Lds<R>& L, int64_t rb, int6
     }
 
     // ------------------------------- peel-off: bins × events (cpu.jl:6-31)
-    constexpr int RCH = ATOMIC ? Shape<R>::kChunkBatch : Shape<R>::kChunk;
+    constexpr int RCH =
+        (ATOMIC && GEO == 0 && SK != kRay) ? Shape<R>::kChunkBatchPlane : Shape<R>::kChunk;
     if (GEO == 1 && !p.rt_cut)
       rt_ranges<RCH, R, GEO, SK, false>(p, L.F, F1, rtrec, esc, slab, E, tid, lane, wave);
     else
