@@ -323,6 +323,13 @@ int mcrt_plan_read_sigma(mcrt_plan* plan, double A_sigma[5], double* Irt_sigma,
  *   PHASE_DIAG  in (cosθ) → (P22, P44) of the peel-off's lookup, then (P22,
  *               P44) as the transport evaluates them (equal to P11, P33 except
  *               for a SplineScatterer, spline.jl:52-55)
+ *   BIN_MATH    in (t, x, y, cosθ) → (e^{-t}, √x, 1/y, acos(cosθ)) as the
+ *               peel-off and CB loops evaluate them (cpu.jl:20, sphere.jl:72,
+ *               utils.jl:92-97, mie.jl:53; the arccos of a Mie or Spline plan's
+ *               table lookup; y a normal number, x ≥ 0)
+ *   CB_PHASE    in (r) → (cos 2πr, sin 2πr) as the CB bins evaluate the
+ *               interference phase ΔΦ = 2πr (cpu.jl:94-99)
+ *   (BIN_MATH and CB_PHASE: the cooperative forms only; -2 for FP64_PLAIN)
  * The table lookups use the scatterer of layer 1 (the plane's upper range and
  * CB, or the sphere's). */
 enum {
@@ -335,7 +342,9 @@ enum {
   MCRT_EVAL_SAMPLE = 6,
   MCRT_EVAL_PHASE_TRANSPORT = 7,
   MCRT_EVAL_KEPLER = 8,
-  MCRT_EVAL_PHASE_DIAG = 9
+  MCRT_EVAL_PHASE_DIAG = 9,
+  MCRT_EVAL_BIN_MATH = 10,
+  MCRT_EVAL_CB_PHASE = 11
 };
 int mcrt_plan_eval(mcrt_plan* plan, int32_t what, int64_t n, const double* in, double* out);
 

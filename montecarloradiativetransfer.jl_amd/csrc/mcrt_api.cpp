@@ -628,9 +628,13 @@ extern "C" int mcrt_plan_read_sigma(mcrt_plan* P, double A_sigma[5], double* Irt
 extern "C" int mcrt_plan_eval(mcrt_plan* P, int32_t what, int64_t n, const double* in, double* out) {
   if (!P) return fail(nullptr, MCRT_ERR_INVALID, "plan is NULL");
   mcrt_ctx* ctx = P->ctx;
-  if (what < MCRT_EVAL_UPDATE_I || what > MCRT_EVAL_PHASE_DIAG)
+  if (what < MCRT_EVAL_UPDATE_I || what > MCRT_EVAL_CB_PHASE)
     return fail(ctx, MCRT_ERR_INVALID, "plan_eval: unknown function %d", what);
   if (n < 0 || (n > 0 && (!in || !out))) return fail(ctx, MCRT_ERR_INVALID, "plan_eval: bad buffers");
+  if (P->precision == MCRT_PRECISION_FP64_PLAIN &&
+      (what == MCRT_EVAL_BIN_MATH || what == MCRT_EVAL_CB_PHASE))
+    return fail(ctx, MCRT_ERR_UNSUPPORTED,
+                "plan_eval: BIN_MATH and CB_PHASE are the cooperative forms' bin-loop functions");
   if (n == 0) return MCRT_OK;
   for (int64_t i = 0; what == MCRT_EVAL_DISTANCE && i < n; ++i) {
     const double idx = in[16 * i + 6];

@@ -1683,6 +1683,15 @@ __global__ void eval_kernel(const CoopParams<R>* __restrict__ pp, int what, int6
     case MCRT_EVAL_KEPLER:
       r[0] = kepler((R)a[0], (R)a[1]);
       break;
+    case MCRT_EVAL_BIN_MATH:
+      r[0] = ExpK<R>::exp_neg((R)(a[0] * ExpK<R>::k));
+      r[1] = bsqrt((R)a[1]);
+      r[2] = recip((R)a[2]);
+      r[3] = t.scale > R(0) ? theta_index(t, (R)a[3]) / t.scale : R(0);
+      break;
+    case MCRT_EVAL_CB_PHASE:
+      phase_cs(a[0], r[0], r[1]);
+      break;
     default:
       break;
   }

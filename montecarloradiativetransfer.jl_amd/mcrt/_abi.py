@@ -171,7 +171,8 @@ class mcrt_server_stats(C.Structure):
 
 # mcrt_plan_eval functions (include/mcrt.h MCRT_EVAL_*)
 EVAL = {"update_I": 0, "update_E": 1, "update_E_path": 2, "distance": 3, "phase": 4,
-        "amplitude": 5, "sample": 6, "phase_transport": 7, "kepler": 8, "phase_diag": 9}
+        "amplitude": 5, "sample": 6, "phase_transport": 7, "kepler": 8, "phase_diag": 9,
+        "bin_math": 10, "cb_phase": 11}
 
 TICKET_STATES = {0: "queued", 1: "running", 2: "done", 3: "cancelled", 4: "failed"}
 

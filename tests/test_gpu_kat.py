@@ -246,3 +246,57 @@ def test_kat_lookups_sampling_kepler(form, kind):
             _check(form, got, want[:, None], "kepler", scale=np.maximum(1, np.abs(want)))
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed", "fp64_plain"])
+def test_kat_bin_math_and_cb_phase(form):
+    """The bin loops' own e^{-t}, √x, 1/y and table arccos (BIN_MATH) and the CB
+    interference phase's cos/sin of 2πr (CB_PHASE) against 30-digit values:
+    within a few ulp in the FP64 form (branch-free polynomials and Newton steps
+    instead of the library functions), within FP32 rounding in the mixed form
+    (hardware exp2/sqrt/rcp/sin/cos, degree-6 arccos).  The plain form has no
+    such loops and refuses both."""
+    import mpmath as mp
+
+    sca, g, cfg = _case("mie", "sphere")
+    plan = mcrt.Plan(cfg, g, [1.0, 0.0, 0.0, 0.0], precision=form)
+    try:
+        n = 1500
+        t = np.concatenate([rng.uniform(0.0, 40.0, n - 4), [0.0, 1e-300, 1e-8, 30.0]])
+        x = np.concatenate([10.0 ** rng.uniform(-30, 4, n - 2), [0.0, 1.0]])
+        y = np.concatenate([10.0 ** rng.uniform(-24, 0.6, n - 1), [1.0]])
+        lim = 0.99 if form == "mixed" else 1.0  # FP32 μ near ±1: arccos ill-conditioned
+        mu = np.concatenate([rng.uniform(-lim, lim, n - 3), [lim, -lim, 0.0]])
+        r = np.concatenate([rng.uniform(-3000.0, 3000.0, 700), rng.uniform(-1.0, 1.0, 700),
+                            [0.0, 0.125, 0.25, 0.5, 0.75, -0.25, -0.5, 1.0]])
+        if form == "fp64_plain":
+            for what, rec in (("bin_math", np.stack([t, x, y, mu], 1)), ("cb_phase", r[:, None])):
+                with pytest.raises(mcrt._abi.MCRTError) as e:
+                    plan.eval(what, rec)
+                assert e.value.code == mcrt._abi.MCRT_ERR_UNSUPPORTED
+            return
+        mp.mp.dps = 30
+        got = plan.eval("bin_math", np.stack([t, x, y, mu], 1))
+        want = np.array([[float(mp.exp(-mp.mpf(a))), float(mp.sqrt(mp.mpf(b))),
+                          float(1 / mp.mpf(c)), float(mp.acos(mp.mpf(d)))]
+                         for a, b, c, d in zip(t, x, y, mu)])
+        rel = np.abs(got[:, :3] - want[:, :3]) / np.where(want[:, :3] == 0, 1.0, np.abs(want[:, :3]))
+        acos_err = np.abs(got[:, 3] - want[:, 3])
+        bounds = (1e-15, 1e-15, 1e-15, 4e-15) if form == "fp64" else (1e-5, 1e-6, 1e-6, 2e-6)
+        for c, (label, err) in enumerate((("exp(-t)", rel[:, 0]), ("sqrt", rel[:, 1]),
+                                          ("1/y", rel[:, 2]), ("acos", acos_err))):
+            w = int(np.argmax(err))
+            print(f"kat {form} bin {label}: max err {err[w]:.3e} (bound {bounds[c]:.0e}) "
+                  f"at {(t[w], x[w], y[w], mu[w])[c]!r}")
+            assert err[w] <= bounds[c], (form, label, err[w])
+        assert np.all(got[x == 0.0, 1] == 0.0)
+        got = plan.eval("cb_phase", r[:, None])
+        want = np.array([[float(mp.cos(2 * mp.pi * mp.mpf(v))), float(mp.sin(2 * mp.pi * mp.mpf(v)))]
+                         for v in r])
+        err = np.abs(got[:, :2] - want).max(axis=1)
+        w = int(np.argmax(err))
+        bound = 2e-15 if form == "fp64" else 1e-5
+        print(f"kat {form} CB phase cos/sin: max abs err {err[w]:.3e} (bound {bound:.0e}) at r = {r[w]!r}")
+        assert err[w] <= bound, (form, err[w])
+    finally:
+        plan.close()
