@@ -302,23 +302,22 @@ std::vector<const mcrt_scatterer*> scatterers_of(const mcrt_geometry* g, int* la
   return v;
 }
 
-// Interleave a scatterer's tables: F per node (F11, F12, F33, F34) for Mie or
-// (F11, F12, F22, F33, F34, F44, 0, 0) for Spline, S per node (S11 re, S11 im,
-// S22 re, S22 im).
+// Interleave a scatterer's tables: F per node (F11, F12, F33, F34) for Mie; for
+// Spline that block over N + 1 nodes (the last zero), then (F22, F44) per node
+// over N + 1 nodes (ScaView); S per node (S11 re, S11 im, S22 re, S22 im).
 template <typename T>
 void interleave(const mcrt_scatterer* s, std::vector<T>& F, std::vector<T>& S, std::vector<T>& csrn) {
   const int N = s->n_theta;
-  const int fs = mcrt::f_stride(s->kind);
-  F.assign((size_t)fs * N, T(0));
+  const bool spl = s->kind == MCRT_SCATTERER_SPLINE;
+  F.assign(spl ? (size_t)mcrt::f_stride(s->kind) * (N + 1) : 4 * (size_t)N, T(0));
   S.resize(4 * (size_t)N);
   csrn.resize((size_t)s->random_points + 1);
   for (int i = 0; i < N; ++i) {
-    T* f = &F[(size_t)fs * i];
-    if (fs == 4) {
-      f[0] = (T)s->F11[i]; f[1] = (T)s->F12[i]; f[2] = (T)s->F33[i]; f[3] = (T)s->F34[i];
-    } else {
-      f[0] = (T)s->F11[i]; f[1] = (T)s->F12[i]; f[2] = (T)s->F22[i];
-      f[3] = (T)s->F33[i]; f[4] = (T)s->F34[i]; f[5] = (T)s->F44[i];
+    T* f = &F[4 * (size_t)i];
+    f[0] = (T)s->F11[i]; f[1] = (T)s->F12[i]; f[2] = (T)s->F33[i]; f[3] = (T)s->F34[i];
+    if (spl) {
+      T* b = &F[(size_t)mcrt::spl_b(N) + 2 * (size_t)i];
+      b[0] = (T)s->F22[i]; b[1] = (T)s->F44[i];
     }
     S[4 * i + 0] = (T)s->S11[2 * i]; S[4 * i + 1] = (T)s->S11[2 * i + 1];
     S[4 * i + 2] = (T)s->S22[2 * i]; S[4 * i + 3] = (T)s->S22[2 * i + 1];

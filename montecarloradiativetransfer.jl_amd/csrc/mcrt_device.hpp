@@ -143,7 +143,9 @@ template <typename R> MCRT_DEV void rotate(R I[4], R c2, R s2) {
 
 // ------------------------------------------------------------ scatterers
 // Device view of a scatterer's tables.  F holds (F11, F12, F33, F34) per node
-// (Mie) or (F11, F12, F22, F33, F34, F44, 0, 0) per node (Spline); S holds
+// (Mie); for a Spline, the same block over Nt + 1 nodes (the last one zero)
+// followed by (F22, F44) per node over Nt + 1 nodes (at spl_b(Nt)), so that a
+// lookup reads 16 + 8 bytes per node with the Mie table's bank spread.  S holds
 // (S11 re, S11 im, S22 re, S22 im) per node; all interleaved.
 template <typename R> struct ScaView {
   int kind;    // 0 Rayleigh, 1 Mie, 2 Spline
@@ -154,8 +156,10 @@ template <typename R> struct ScaView {
   const R* F;
   const R* S;
 };
-// Words per node of a tabulated scatterer's F table.
-MCRT_HD int f_stride(int kind) { return kind == 2 ? 8 : 4; }
+// Words per node of a tabulated scatterer's F table, and the offset of a
+// Spline table's (F22, F44) block.
+MCRT_HD int f_stride(int kind) { return kind == 2 ? 6 : 4; }
+MCRT_HD int spl_b(int Nt) { return 4 * (Nt + 1); }
 
 // Inverse-CDF sample of cosθ from the uniform u (rayleigh.jl:9-11, mie.jl:47-51,
 // spline.jl:27-31: the same csrn lookup for Mie and Spline).
@@ -211,13 +215,14 @@ template <typename R> MCRT_DEV void phase_eval(const ScaView<R>& s, R mu, R P[6]
     return;
   }
   R w0 = R(i0 + 2) - sn;
-  const R* a = s.F + 8 * i0;
-#pragma unroll
-  for (int c = 0; c < 6; ++c) P[c] = w0 * a[c];
+  const R* a = s.F + 4 * i0;
+  const R* b = s.F + spl_b(s.Nt) + 2 * i0;
+  P[0] = w0 * a[0]; P[1] = w0 * a[1]; P[2] = w0 * b[0];
+  P[3] = w0 * a[2]; P[4] = w0 * a[3]; P[5] = w0 * b[1];
   if (i0 + 1 <= s.Nt - 1) {
     R w1 = sn - R(i0 + 1);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) P[c] += w1 * a[8 + c];
+    P[0] += w1 * a[4]; P[1] += w1 * a[5]; P[2] += w1 * b[2];
+    P[3] += w1 * a[6]; P[4] += w1 * a[7]; P[5] += w1 * b[3];
   }
 }
 
