@@ -93,6 +93,15 @@ def _case(name):
         g = PlaneGeometry(math.radians(140.0), math.radians(70.0),
                           [Layer(d=sp.kl, scatterer=sp, tau=1.0, omega=sp.omega)])
         return Config(g, number_of_rays=1024, Ntheta=60, Nphi=24, compute_cb=True), g, 35, (1.0, 0.0, 0.0, 1.0)
+    if name == "plane_three_scatterers_cb":
+        # a middle layer whose scatterer is neither table set's: the transport's
+        # per-layer tables alone (scatterer(g, idx), cpu.jl:206-234, plane.jl:46-50)
+        r, m, sp = RayleighScatterer(0.95, 1.0), _mie(), _spline()
+        g = PlaneGeometry(math.radians(145.0), math.radians(30.0),
+                          [Layer(d=0.3, scatterer=r, tau=0.3, omega=0.95),
+                           Layer(d=0.8 * m.kl, scatterer=m, tau=0.8, omega=m.omega),
+                           Layer(d=0.6 * sp.kl, scatterer=sp, tau=0.6, omega=sp.omega)])
+        return Config(g, number_of_rays=1024, Ntheta=60, Nphi=24, compute_cb=True), g, 37, (1.0, -1.0, 0.0, 0.0)
     if name == "sphere_mie_2401_cb":
         # beyond the default layout: S from HBM (mixed), 64 slots (fp64)
         s = MieScatterer.create(0.3, complex(1.45, 0.01), 0.55, 0.1, Ntheta=2401, random_points=1000)
@@ -109,7 +118,7 @@ def _case(name):
 CASES = ["sphere_rayleigh_cb", "plane_mie_oblique_cb", "plane_two_layers", "plane_semi_infinite",
          "sphere_mie_tau0_cut", "sphere_short_paths", "plane_rayleigh_over_mie_cb",
          "plane_mie_over_rayleigh_cb", "plane_rayleigh_over_spline", "sphere_spline_cb",
-         "plane_spline_oblique_cb", "sphere_mie_2401_cb"]
+         "plane_spline_oblique_cb", "plane_three_scatterers_cb", "sphere_mie_2401_cb"]
 _ORACLE = {}
 
 
