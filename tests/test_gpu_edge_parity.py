@@ -107,6 +107,12 @@ def _case(name):
         s = MieScatterer.create(0.3, complex(1.45, 0.01), 0.55, 0.1, Ntheta=2401, random_points=1000)
         g = SphereGeometry(R=2.0 * s.kl, scatterer=s)
         return Config(g, number_of_rays=512, Ntheta=60, Nphi=16, compute_cb=True), g, 36, (1.0, -1.0, 0.0, 0.0)
+    if name == "sphere_spline_2401_cb":
+        # a Spline table beyond the default layout: S from HBM (mixed), 64 slots (fp64)
+        sp = mcrt.workloads.spline_from(
+            MieScatterer.create(0.3, complex(1.45, 0.01), 0.55, 0.1, Ntheta=2401, random_points=1000))
+        g = SphereGeometry(R=2.0 * sp.kl, scatterer=sp)
+        return Config(g, number_of_rays=512, Ntheta=60, Nphi=16, compute_cb=True), g, 38, (1.0, 0.0, 1.0, 0.0)
     if name == "sphere_short_paths":
         s = _mie()
         g = SphereGeometry(R=4.0 * s.kl, scatterer=s, tau=4.0, omega=s.omega)
@@ -118,7 +124,8 @@ def _case(name):
 CASES = ["sphere_rayleigh_cb", "plane_mie_oblique_cb", "plane_two_layers", "plane_semi_infinite",
          "sphere_mie_tau0_cut", "sphere_short_paths", "plane_rayleigh_over_mie_cb",
          "plane_mie_over_rayleigh_cb", "plane_rayleigh_over_spline", "sphere_spline_cb",
-         "plane_spline_oblique_cb", "plane_three_scatterers_cb", "sphere_mie_2401_cb"]
+         "plane_spline_oblique_cb", "plane_three_scatterers_cb", "sphere_mie_2401_cb",
+         "sphere_spline_2401_cb"]
 _ORACLE = {}
 
 
@@ -211,6 +218,20 @@ def test_lds_layout_follows_the_table_size():
     assert shapes[(1801, "fp64")]["slots"] == 128
     assert shapes[(2401, "fp64")]["slots"] == 64
     assert shapes[(4001, "fp64")] is None  # beyond 64 FP64 slots
+    # a Spline table: (F11, F12, F33, F34) and (F22, F44) blocks, 24 (mixed) or 48 (fp64)
+    # bytes per node, no csrn in LDS (the general-kind transport reads it from HBM)
+    sp = mcrt.workloads.spline_from(
+        MieScatterer.create(0.3, complex(1.45, 0.01), 0.55, 0.1, Ntheta=2401, random_points=1000))
+    g = SphereGeometry(R=1.0 * sp.kl, scatterer=sp)
+    cfg = Config(g, number_of_rays=1 << 17, Ntheta=60, Nphi=16, compute_cb=True)
+    sh = {}
+    for form in ("mixed", "fp64"):
+        p = mcrt.Plan(cfg, g, I0, seed=3, precision=form)
+        sh[form] = p.shape()
+        p.close()
+    print("spline 2401", sh)
+    assert sh["mixed"]["slots"] == 256 and sh["mixed"]["stage_s"] == -1
+    assert sh["fp64"]["slots"] == 64
     big = MieScatterer.create(0.3, complex(1.45, 0.01), 0.55, 0.1, Ntheta=12001, random_points=1000)
     g = SphereGeometry(R=1.0 * big.kl, scatterer=big)
     cfg = Config(g, number_of_rays=512, Ntheta=60, Nphi=16, compute_cb=True)
