@@ -912,14 +912,14 @@ MCRT_DEV void cb_pass(const CoopParams<R>& p, const R* F, const R* S, const R* c
         s0 = Ia0 * recip(fmax_r(Ia0 + Ib0, std::numeric_limits<R>::min()));
         s12 = s0;
       }
-      // the final rotation (cpu.jl:118-124) is linear, so it acts before the
-      // scale factors, which then fold into the accumulating FMAs
-      rotate(o, b[8], b[9]);
       acc[0] = fma_r(o[0], s0, acc[0]);
       acc[1] = fma_r(o[1], s12, acc[1]);
       acc[2] = fma_r(o[2], s12, acc[2]);
       acc[3] = fma_r(o[3], s0, acc[3]);
     }
+    // the final rotation (cpu.jl:118-124) is linear in (Q, U) with the bin's own
+    // coefficients, and Q and U share the scale s12: it acts once on the sums
+    rotate(acc, b[8], b[9]);
     const int nit = it + kThreads;
     Item nxt;
     if (nit < total) load(nit, nxt);
