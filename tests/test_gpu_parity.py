@@ -70,6 +70,34 @@ def test_form_vs_oracle(case, form):
     assert abs(st["events"] - ev_ref) <= 0.02 * ev_ref + 5
 
 
+_BIG = []
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_headline_workload_at_16384_rays(form):
+    """The bench's own workload (sphere τ = 5 of Mie spheres, Mie table of 1801
+    nodes, CB on) at 16384 rays of one stream in every form against the oracle:
+    32× the rays of the case above, so each bin's σ is ~6× smaller and the same
+    same-stream criteria are that much tighter in absolute terms."""
+    cfg, g, seed = W.sphere_mie_cb(16384, Ntheta_table=1801)
+    I0 = [1.0, -1.0, 0.0, 0.0]
+    print(f"case=sphere_mie_cb form={form} N={cfg.number_of_rays}", flush=True)
+    if not _BIG:
+        _BIG.append(oracle.Solve(cfg, g).simulate(I0, seed, nbatch=32))
+    ref = _BIG[0]
+    st = {}
+    out = mcrt.simulate(cfg, g, I0, seed=seed, precision=form, stats=st)
+    assert abs(sum(out[:5]) - 1.0) < (1e-5 if form == "mixed" else 1e-9)
+    assert st["rays"] == cfg.number_of_rays and st["nonfinite"] == 0
+    r = compare(flat(out), oracle_flat(ref), oracle.sigma_of_mean(ref) * np.sqrt(2.0),
+                floor=1e-15, dead_rel=1e-6 if form == "mixed" else 1e-12,
+                label=f"sphere_mie_cb 16384/{form}")
+    assert r["ok"], r
+    assert r["max_abs_z"] < (1.0 if form == "mixed" else 0.5), r
+    ev_ref = ref["counters"][0]
+    assert abs(st["events"] - ev_ref) <= 0.02 * ev_ref + 5
+
+
 @pytest.mark.parametrize("form", ["fp64", "mixed"])
 def test_independent_streams_agree_within_3sigma(form):
     """A genuinely statistical check: the GPU form on one Philox key against the
