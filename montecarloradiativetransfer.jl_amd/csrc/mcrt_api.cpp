@@ -459,8 +459,8 @@ extern "C" int mcrt_plan_create_shaped(mcrt_ctx* ctx, const mcrt_config* cfg,
   P->precision = precision;
   P->N = cfg->number_of_rays;
   P->geo_kind = geom->kind;
-  P->nrt = 4 * cfg->n_theta * cfg->n_phi;
-  P->ncb = 4 * cfg->n_theta_b * cfg->n_phi_b;
+  P->nrt = 4 * (int64_t)cfg->n_theta * cfg->n_phi;
+  P->ncb = 4 * (int64_t)cfg->n_theta_b * cfg->n_phi_b;
   hipError_t e;
   P->dA = static_cast<double*>(P->alloc(sizeof(double) * 5, &e));
   if (P->dA) P->dIrt = static_cast<double*>(P->alloc(sizeof(double) * P->nrt, &e));

@@ -61,6 +61,7 @@ constexpr int kCbRec = 40;  // R words of a CB event record
 constexpr int kCbDx = 4;    // doubles of a CB event record (x_a - x_b, τab)
 constexpr int kBinW = 11;     // R words of a bin used by the evaluation
 constexpr int kBinStride = 12;  // R words per bin in the bin tables
+constexpr int64_t kMaxCoopBins = int64_t(1) << 26;  // bins per grid (RT or CB)
 constexpr int kAcosTerms = 7;   // coefficients of the FP32 arccos polynomial (degree 6)
 
 // Scatterer kind of a kernel instantiation: Rayleigh, Mie or Spline tables for
@@ -2113,6 +2114,14 @@ int init_params(CoopParams<R>& P, size_t* lds_out, size_t lds_max, const mcrt_co
                 std::string* err, TableCache* cache = nullptr,
                 std::vector<const void*>* refs = nullptr, int max_slots = 0, bool want_s = true) {
   std::memset(&P, 0, sizeof P);
+  // bin offsets are 32-bit in the kernels (kBinStride·bin < 2³¹), and the
+  // private FP64 slabs of a larger grid would not fit in device memory
+  if ((int64_t)cfg->n_theta * cfg->n_phi > kMaxCoopBins ||
+      (cfg->compute_cb && (int64_t)cfg->n_theta_b * cfg->n_phi_b > kMaxCoopBins)) {
+    *err = "bin grids of more than 2^26 bins are not supported by the cooperative forms "
+           "(use fp64_plain)";
+    return MCRT_ERR_UNSUPPORTED;
+  }
   const bool plane = g->kind == MCRT_GEOMETRY_PLANE;
   const mcrt_scatterer* s0 = plane ? g->layers[0].scatterer : g->scatterer;
   const mcrt_scatterer* s1 = plane ? g->layers[g->n_layers - 1].scatterer : s0;
@@ -2514,7 +2523,8 @@ size_t solve_bytes_estimate(const mcrt_config* cfg, const mcrt_geometry* g, bool
   const int64_t cr =
       chunk_rays > 0 ? chunk_rays : default_chunk_rays(N, solve_work_per_ray(cfg, g), rule);
   const int64_t nb = std::min<int64_t>(kBatchSlabs, (N + cr - 1) / cr);
-  const int64_t nbins = cfg->n_theta * cfg->n_phi + (cfg->compute_cb ? cfg->n_theta_b * cfg->n_phi_b : 0);
+  const int64_t nbins = (int64_t)cfg->n_theta * cfg->n_phi +
+                        (cfg->compute_cb ? (int64_t)cfg->n_theta_b * cfg->n_phi_b : 0);
   const int64_t nout = 5 + 4 * nbins;
   const size_t r = mixed ? 4 : 8;
   return (size_t)(nb + 2) * (size_t)nout * 8 + (size_t)nbins * kBinStride * r +

@@ -243,6 +243,23 @@ def test_lds_layout_follows_the_table_size():
 
 
 @pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_bin_grid_beyond_the_cooperative_limit_is_refused(form):
+    """An RT grid of 8192 × 8193 bins (just over 2^26) or a CB grid of that size:
+    the cooperative forms refuse it before building anything, where 32-bit bin
+    offsets could otherwise wrap (fp64_plain takes any grid that fits in memory)."""
+    g = SphereGeometry(R=1.0, scatterer=RayleighScatterer(1.0, 1.0))
+    I0 = [1.0, -1.0, 0.0, 0.0]
+    for kw in (dict(Ntheta=8192, Nphi=8193, compute_cb=False),
+               dict(Ntheta=8, Nphi=8, compute_cb=True, thetab=np.linspace(0.0, 0.1, 8192),
+                    Nphib=8193)):
+        cfg = Config(g, number_of_rays=64, **kw)
+        print(f"form={form} grid={cfg.Ntheta}x{cfg.Nphi} cb={cfg.Nthetab}x{cfg.Nphib}", flush=True)
+        with pytest.raises(mcrt._abi.MCRTError) as e:
+            mcrt.Plan(cfg, g, I0, seed=1, precision=form)
+        assert e.value.code == mcrt._abi.MCRT_ERR_UNSUPPORTED and "2^26" in str(e.value)
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
 @pytest.mark.parametrize("case", ["plane_rayleigh_over_mie_cb", "plane_mie_over_rayleigh_cb",
                                   "sphere_spline_cb", "plane_two_layers", "sphere_rayleigh_cb"])
 def test_single_ray_histories_match_the_oracle(case, form):
